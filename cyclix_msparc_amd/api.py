@@ -1,0 +1,372 @@
+"""Host-side mirror of the reference's hot-path interface, over the C-ABI.
+
+Names, argument meaning and error behaviour follow the MATLAB routines they
+replace (paths relative to the reference root ``Cyclix_MS_200220/``):
+
+* ``h_nonlocal_vector_mult``  - ``src/h_nonlocal_vector_mult.m:1``
+* ``chebyshev_filter``        - ``src/chebyshev_filter.m:1``
+* ``eigSolver``               - ``src/eigSolver.m:1`` (serial channel loop)
+* ``poissonSolve``            - ``src/poissonSolve.m:1``
+* ``aar``                     - ``src/aar.m:1``
+
+The sparse 1-D factors (DL11..DG3), ``S.Lap_std`` and the ILU factors that the
+MATLAB signatures carry are replaced by a device-resident operator built from
+the grid scalars (``Hamiltonian`` / ``Poisson``); the Jacobi diagonal stands in
+for ILU.  Blocks of states are torch tensors on the GPU of shape (ns, N): row
+s is column s of MATLAB's N x ns ``psi`` (the same bytes, column-major).
+torch is only the allocator and stream here; every operation runs in the
+HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as nat
+from .system import CyclixSystem, GridSpec, ProjAtom
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+TORCH_DTYPES = {}
+if torch is not None:
+    TORCH_DTYPES = {nat.CYX_F64: torch.float64, nat.CYX_C128: torch.complex128,
+                    nat.CYX_F32: torch.float32, nat.CYX_C64: torch.complex64}
+
+
+def grid_desc(g: GridSpec) -> nat.GridDesc:
+    return nat.GridDesc(g.nr, g.nt, g.nz, g.fdn, g.cell_typ, 0, g.xin, g.dr, g.dt, g.dz, g.twist)
+
+
+class ProjectorPack:
+    """``S.Atom(J).rcImage(img)`` flattened into a cyx_projector_desc (the
+    numpy arrays are kept alive with the descriptor)."""
+
+    def __init__(self, atoms: list[ProjAtom]):
+        self.angnum = np.array([a.gamma.size for a in atoms], dtype=np.int32)
+        self.gamma = np.ascontiguousarray(np.concatenate([a.gamma for a in atoms]) if atoms else np.zeros(0))
+        img_atom, nn, disp, pos, chi = [], [], [], [], []
+        for J, a in enumerate(atoms):
+            for im in a.images:
+                img_atom.append(J)
+                nn.append(im.rc_pos.size)
+                disp.append(a.coords - im.coords)
+                pos.append(im.rc_pos.astype(np.int32))
+                chi.append(np.asarray(im.chi, dtype=float).ravel(order="F"))
+        self.image_atom = np.array(img_atom, dtype=np.int32)
+        self.image_nnodes = np.array(nn, dtype=np.int32)
+        self.image_disp = np.ascontiguousarray(np.array(disp, dtype=float).reshape(-1))
+        self.rc_pos = np.ascontiguousarray(np.concatenate(pos) if pos else np.zeros(0, np.int32))
+        self.chi = np.ascontiguousarray(np.concatenate(chi) if chi else np.zeros(0))
+        self.desc = nat.ProjectorDesc(len(atoms), len(img_atom), nat.iptr(self.angnum), nat.dptr(self.gamma),
+                                      nat.iptr(self.image_atom), nat.iptr(self.image_nnodes),
+                                      nat.dptr(self.image_disp), nat.iptr(self.rc_pos), nat.dptr(self.chi))
+
+
+class Device:
+    """A HIP context bound to torch's current stream on ``device``."""
+
+    def __init__(self, device: int = 0):
+        if torch is None or not torch.cuda.is_available():
+            raise RuntimeError("cyclix_msparc_amd needs a ROCm GPU (torch.cuda is unavailable)")
+        self.lib = nat.load()
+        self.index = device
+        h = C.c_void_p()
+        nat.check(self.lib.cyx_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.torch_device = torch.device("cuda", device)
+        stream = torch.cuda.current_stream(self.torch_device)
+        nat.check(self.lib.cyx_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    def synchronize(self):
+        nat.check(self.lib.cyx_ctx_synchronize(self.h))
+
+    PHASES = ("stencil", "h_apply", "kb_gather", "gram", "rotate", "wnorm", "dense_eig", "aar")
+
+    def set_timing(self, on=True):
+        nat.check(self.lib.cyx_ctx_set_timing(self.h, 1 if on else 0))
+
+    def reset_timing(self):
+        nat.check(self.lib.cyx_ctx_reset_timing(self.h))
+
+    def timing(self):
+        """{phase: (total_ms, launches)} from the device event timers."""
+        out = {}
+        for ph in self.PHASES:
+            ms, n = C.c_double(), C.c_int64()
+            nat.check(self.lib.cyx_ctx_read_timing(self.h, ph.encode(), C.byref(ms), C.byref(n)))
+            out[ph] = (ms.value, n.value)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.cyx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dtype_code(t) -> int:
+    for k, v in TORCH_DTYPES.items():
+        if v == t.dtype:
+            return k
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+def _check_block(t, N):
+    if not (t.is_cuda and t.is_contiguous() and t.dim() == 2 and t.shape[1] == N):
+        raise ValueError(f"expected a contiguous CUDA block of shape (ns, {N}), got {tuple(t.shape)}")
+
+
+class Hamiltonian:
+    """Device operator of one Bloch channel (``blochLaplacian_1d`` +
+    ``S.Veff`` + ``S.Atom`` projectors)."""
+
+    def __init__(self, dev: Device, system: CyclixSystem, kpt, veff, projectors=None):
+        self.dev = dev
+        self.lib = dev.lib
+        self.system = system
+        self.g = system.grid
+        self.N = self.g.N
+        self.kpt = np.ascontiguousarray(np.asarray(kpt, dtype=float))
+        self.complex = bool(self.kpt[1] != 0 or self.kpt[2] != 0)
+        veff = np.ascontiguousarray(np.asarray(veff.cpu() if hasattr(veff, "cpu") else veff, dtype=float))
+        W = np.ascontiguousarray(system.W, dtype=float)
+        self._gd = grid_desc(self.g)
+        self._proj = ProjectorPack(projectors) if projectors else None
+        h = C.c_void_p()
+        nat.check(self.lib.cyx_ham_create(dev.h, C.byref(self._gd), nat.dptr(self.kpt), nat.dptr(veff), nat.dptr(W),
+                                          C.byref(self._proj.desc) if self._proj else None, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cyx_ham_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_veff(self, veff):
+        if hasattr(veff, "is_cuda") and veff.is_cuda:
+            v = veff.contiguous().to(torch.float64)
+            nat.check(self.lib.cyx_ham_set_veff(self.h, C.c_void_p(v.data_ptr()), 1))
+        else:
+            v = np.ascontiguousarray(np.asarray(veff, dtype=float))
+            nat.check(self.lib.cyx_ham_set_veff(self.h, v.ctypes.data_as(C.c_void_p), 0))
+
+    def new_block(self, ns, dtype_code):
+        return torch.empty((ns, self.N), dtype=TORCH_DTYPES[dtype_code], device=self.dev.torch_device)
+
+    def apply(self, X):
+        """HX = H X  (``src/h_nonlocal_vector_mult.m:21-43``)."""
+        _check_block(X, self.N)
+        HX = torch.empty_like(X)
+        nat.check(self.lib.cyx_h_apply(self.h, _dtype_code(X), C.c_void_p(X.data_ptr()), C.c_void_p(HX.data_ptr()),
+                                       X.shape[0], self.N))
+        return HX
+
+    def chebyshev_filter(self, X, m, a, b, a0, overwrite=False):
+        """``src/chebyshev_filter.m:24-42``; X is preserved unless overwrite."""
+        _check_block(X, self.N)
+        Xw = X if overwrite else X.clone()
+        Y = torch.empty_like(X)
+        nat.check(self.lib.cyx_chebyshev_filter(self.h, _dtype_code(X), C.c_void_p(Xw.data_ptr()),
+                                                C.c_void_p(Y.data_ptr()), X.shape[0], self.N, int(m), float(a),
+                                                float(b), float(a0)))
+        return Y
+
+    def eig_pass(self, psi, npl, lambda_cutoff, bup, a0, work=None):
+        """One CheFSI pass in place on psi; returns the ascending Ritz values
+        (``src/eigSolver.m:98-124``)."""
+        _check_block(psi, self.N)
+        ns = psi.shape[0]
+        if work is None:
+            work = torch.empty_like(psi)
+        ev = np.zeros(ns)
+        nat.check(self.lib.cyx_eigsolver_pass(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()),
+                                              C.c_void_p(work.data_ptr()), ns, int(npl), float(lambda_cutoff),
+                                              float(bup), float(a0), nat.dptr(ev)))
+        return ev
+
+    def rayleigh_ritz(self, psi, work=None):
+        _check_block(psi, self.N)
+        ns = psi.shape[0]
+        if work is None:
+            work = torch.empty_like(psi)
+        ev = np.zeros(ns)
+        nat.check(self.lib.cyx_rayleigh_ritz(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()),
+                                             C.c_void_p(work.data_ptr()), ns, nat.dptr(ev)))
+        return ev
+
+    def density_accumulate(self, rho, psi, occ, weight=1.0):
+        """rho += 2 w sum_s occ_s |psi_s|^2  (``src/electronDensity.m:13-17``)."""
+        _check_block(psi, self.N)
+        occ = np.ascontiguousarray(np.asarray(occ, dtype=float))
+        nat.check(self.lib.cyx_density_accumulate(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()), psi.shape[0],
+                                                  nat.dptr(occ), float(weight), C.c_void_p(rho.data_ptr())))
+        return rho
+
+    def energy_terms(self, rho, vxc, phi, b, rhotol=1e-14):
+        """(Exc, Exc_dc, Eelec_dc) grid sums (``src/evaluateTotalEnergy.m:27-52,72``)."""
+        out = np.zeros(3)
+        nat.check(self.lib.cyx_energy_terms(self.h, C.c_void_p(rho.data_ptr()), C.c_void_p(vxc.data_ptr()),
+                                            C.c_void_p(phi.data_ptr()), C.c_void_p(b.data_ptr()), float(rhotol),
+                                            nat.dptr(out)))
+        return tuple(out)
+
+    def lanczos_bounds(self, v0, maxit=300, tol=1e-3):
+        v0 = np.ascontiguousarray(np.asarray(v0, dtype=float))
+        lo, hi = C.c_double(), C.c_double()
+        nat.check(self.lib.cyx_lanczos_bounds(self.h, nat.dptr(v0), int(maxit), float(tol), C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+
+def lda_pw(dev: Device, rho, phi=None, rhotol=1e-14):
+    """(vxc, veff): LDA-PW potential (``src/exchangeCorrelationPotential.m:82-106``)
+    and, with phi, Veff = phi + Vxc (``src/scf.m:39``); device FP64 vectors."""
+    vxc = torch.empty_like(rho)
+    veff = torch.empty_like(rho) if phi is not None else None
+    nat.check(dev.lib.cyx_lda_pw(dev.h, rho.numel(), C.c_void_p(rho.data_ptr()),
+                                 C.c_void_p(phi.data_ptr()) if phi is not None else None,
+                                 C.c_void_p(vxc.data_ptr()), C.c_void_p(veff.data_ptr()) if veff is not None else None,
+                                 float(rhotol)))
+    return vxc, veff
+
+
+class Poisson:
+    """``S.Lap_std`` (k = 0) with a Jacobi preconditioner, on the device."""
+
+    def __init__(self, dev: Device, grid: GridSpec):
+        self.dev = dev
+        self.lib = dev.lib
+        self.N = grid.N
+        self._gd = grid_desc(grid)
+        h = C.c_void_p()
+        nat.check(self.lib.cyx_poisson_create(dev.h, C.byref(self._gd), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cyx_poisson_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def aar(self, f, x_guess=None, tol=1e-8, max_iter=1000, omega=0.6, beta=0.6, m=7, p=6):
+        """``src/aar.m:33-72``; returns (x, info).  Raises on non-convergence
+        only through ``info.converged`` (the reference prints and returns)."""
+        f = f.contiguous()
+        x = x_guess.clone().contiguous() if x_guess is not None else torch.empty_like(f)
+        info = nat.AarInfo()
+        nat.check(self.lib.cyx_aar(self.h, C.c_void_p(f.data_ptr()), C.c_void_p(x.data_ptr()),
+                                   1 if x_guess is not None else 0, float(tol), int(max_iter), float(omega),
+                                   float(beta), int(m), int(p), C.byref(info)), allow=(nat.CYX_ERR_NOCONV,))
+        return x, info
+
+    def solve(self, rho, b, phi_guess=None, tol=1e-8, max_iter=1000):
+        """``src/poissonSolve.m:17-36`` (Cyclix branch)."""
+        phi = phi_guess.clone().contiguous() if phi_guess is not None else torch.empty_like(rho)
+        info = nat.AarInfo()
+        nat.check(self.lib.cyx_poisson_solve(self.h, C.c_void_p(rho.contiguous().data_ptr()),
+                                             C.c_void_p(b.contiguous().data_ptr()), C.c_void_p(phi.data_ptr()),
+                                             1 if phi_guess is not None else 0, float(tol), int(max_iter),
+                                             C.byref(info)), allow=(nat.CYX_ERR_NOCONV,))
+        return phi, info
+
+
+# --------------------------------------------------------------------------
+# reference-named entry points
+# --------------------------------------------------------------------------
+def h_nonlocal_vector_mult(ham: Hamiltonian, X):
+    """Hnlx = h_nonlocal_vector_mult(DL11,DL22,DL33,DG1,DG2,DG3,Veff,X,S,kptvec);
+    the channel operator (1-D factors, Veff, S, kptvec) is ``ham``."""
+    return ham.apply(X)
+
+
+def chebyshev_filter(X, m, a, b, a0, ham: Hamiltonian):
+    """Y = chebyshev_filter(X,m,a,b,a0,DL11,DL22,DL33,DG1,DG2,DG3,Veff,S,kpt_vec)."""
+    return ham.chebyshev_filter(X, m, a, b, a0)
+
+
+@dataclass
+class EigState:
+    """The fields of ``S`` that eigSolver reads and writes."""
+
+    hams: list                    # one Hamiltonian per channel (S.kptgrid rows)
+    npl: int                      # S.npl
+    Nev: int                      # S.Nev
+    EigVal: np.ndarray = None     # (Nev, nch), S.EigVal
+    elecgs_Count: int = 1
+    rhoTrigger: int = 4
+    chefsibound_flag: int = 0
+    TOL_LANCZOS: float = 1e-2
+    lanczos_maxit: int = 300
+    work: list = field(default_factory=list)
+
+
+def eigSolver(S: EigState, count, upper_bound_guess_vecs, psi, EigVal, a0, bup, lambda_cutoff):
+    """[ubgv,psi,EigVal,a0,bup,lambda_cutoff] = eigSolver(S,count,...)
+    ``src/eigSolver.m:26-124`` (serial branch).  psi: list of device blocks
+    (one per channel), updated in place.  The spectral bounds use the
+    library's Lanczos estimator (``cyx_lanczos_bounds``) in place of eigs."""
+    if S.elecgs_Count > 1:
+        S.rhoTrigger = 1
+    nch = len(S.hams)
+    for ks in range(nch):
+        ham = S.hams[ks]
+        if count == 1:
+            v0 = np.random.default_rng(ks + 1).random(ham.N)
+            lo, hi = ham.lanczos_bounds(v0, S.lanczos_maxit, S.TOL_LANCZOS)
+            bup[ks] = 1.01 * hi
+            upper_bound_guess_vecs[ks] = None
+            a0[ks] = (lo - 0.1) if S.elecgs_Count == 1 else np.min(S.EigVal[:, ks])
+        elif count > S.rhoTrigger:
+            if S.chefsibound_flag == 1:
+                v0 = np.random.default_rng(ks + 1).random(ham.N)
+                _, hi = ham.lanczos_bounds(v0, S.lanczos_maxit, S.TOL_LANCZOS)
+                bup[ks] = 1.01 * hi
+            a0[ks] = np.min(S.EigVal[:, ks])
+        if count == 1 and S.elecgs_Count == 1:
+            lambda_cutoff[ks] = 0.5 * (bup[ks] + a0[ks])
+        else:
+            lambda_cutoff[ks] = np.max(S.EigVal[:, ks]) + 0.10
+        work = S.work[ks] if ks < len(S.work) else None
+        EigVal[:, ks] = ham.eig_pass(psi[ks], S.npl, lambda_cutoff[ks], bup[ks], a0[ks], work=work)
+    S.EigVal = EigVal
+    return upper_bound_guess_vecs, psi, EigVal, a0, bup, lambda_cutoff
+
+
+def aar(A: Poisson, b, x_guess, tol, max_iter, omega, beta, m, p, L=None, U=None):
+    """x = aar(A,b,x_guess,tol,max_iter,omega,beta,m,p,L,U); ``A`` is the
+    device Poisson operator, the preconditioner is its Jacobi diagonal (L, U
+    are accepted and ignored).  Prints the reference's convergence message."""
+    x, info = A.aar(b, x_guess, tol, max_iter, omega, beta, m, p)
+    if not info.converged:
+        print(f" AAR exceeded maximum iterations and converged to a relative residual of {info.relres:g}. ")
+    return x
+
+
+def poissonSolve(S, poisson_tol, Isguess):
+    """S = poissonSolve(S,poisson_tol,Isguess) for Cyclix: S.phi from
+    f = -4 pi (S.rho + S.b) (``src/poissonSolve.m:19-36``).  ``S`` carries
+    ``poisson`` (a Poisson operator), ``rho``, ``b``, ``phi`` (device
+    vectors) and ``MAXIT_POISSON``."""
+    guess = S.phi if Isguess else None
+    S.phi, S.poisson_info = S.poisson.solve(S.rho, S.b, guess, poisson_tol, getattr(S, "MAXIT_POISSON", 1000))
+    return S

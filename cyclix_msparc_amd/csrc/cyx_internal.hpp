@@ -1,0 +1,153 @@
+// Internal (non-ABI) declarations shared by the library's translation units.
+#pragma once
+#include <vector>
+
+#include "../../include/cyclix_msparc.h"
+#include "cyx_common.hpp"
+
+struct rocblas_handle_s;
+
+namespace cyx {
+
+constexpr int FDN = 6;            // finite-difference half-width the kernels are built for
+constexpr int QN = 2 * FDN + 1;   // z queue depth of the marching stencil
+constexpr int RCOEF = 2 * FDN + 2;  // per-radius coefficients: c+[6], c-[6], c_theta, c_diag
+
+// Per-channel operator coefficients in one precision (R = double or float).
+template <class R>
+struct OpCoef {
+  R* rcoef = nullptr;   // [nr][RCOEF]   (-1/2 folded in)
+  R* veff = nullptr;    // [N]
+  R w2[FDN + 1];        // FD weights of the second derivative
+  R w1[FDN + 1];        // FD weights of the first derivative
+  R s_r2, s_r1, s_z2;   // -1/2/dr^2, -1/2/dr (times 1/r), -1/2/dz^2
+  R s_x;                // (-2 tau/dt)(-1/2/dz): helical cross term
+};
+
+// KB tables in one arithmetic type T (complex phases for Bloch channels).
+template <class T>
+struct KbTables {
+  int nproj = 0;
+  // gather, sorted by projector: C[s][p] = gamma_p sum_e g[e] * Y[node[e]]
+  int* g_ptr = nullptr;   // [nproj+1]
+  int* g_node = nullptr;  // [nent]
+  T* g_coef = nullptr;    // [nent]  chi * W * e^{+i k.d}
+  typename traits<T>::real* gamma = nullptr;  // [nproj]
+  // scatter, CSR by node: HY[n] += sum_e s[e] * C[s][proj[e]]
+  int* s_ptr = nullptr;   // [N+1]
+  int* s_proj = nullptr;  // [nent]
+  T* s_coef = nullptr;    // [nent]  chi * e^{-i k.d}
+};
+
+struct DeviceBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace cyx
+
+namespace cyx {
+// Phases timed with HIP events on the context's stream when timing is on.
+enum Phase { PH_STENCIL = 0, PH_HAPPLY, PH_KB_GATHER, PH_GRAM, PH_ROTATE, PH_WNORM, PH_DENSE_EIG, PH_AAR, PH_COUNT };
+}
+
+struct cyx_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* blas = nullptr;  // rocblas_handle
+  bool timing = false;
+  struct Mark {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;            // recorded, not yet resolved
+  double phase_ms[cyx::PH_COUNT] = {};
+  int64_t phase_count[cyx::PH_COUNT] = {};
+};
+
+namespace cyx {
+// RAII event pair around one device phase (no-op when timing is off)
+struct PhaseTimer {
+  cyx_ctx_s* c;
+  int phase;
+  hipEvent_t a = nullptr;
+  PhaseTimer(cyx_ctx_s* ctx, int ph) : c(ctx), phase(ph) {
+    if (c->timing && hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, c->stream);
+  }
+  ~PhaseTimer() {
+    if (c->timing && a) {
+      hipEvent_t b = nullptr;
+      if (hipEventCreate(&b) == hipSuccess) {
+        (void)hipEventRecord(b, c->stream);
+        c->marks.push_back({phase, a, b});
+      }
+    }
+  }
+};
+}  // namespace cyx
+
+struct cyx_ham_s {
+  cyx_ctx_s* ctx = nullptr;
+  cyx_grid_desc g{};
+  double kpt[3] = {0, 0, 0};
+  bool bloch = false;     // any non-zero Bloch phase (requires complex dtypes)
+  double* W = nullptr;    // [N] FP64 quadrature weights S.W (normalisation, KB gather)
+  double* Wm = nullptr;   // [N] metric weights r dr dt dz (H is self-adjoint in them)
+  cyx::OpCoef<double> c64;
+  cyx::OpCoef<float> c32;
+  // phases e^{-i nu L2}, e^{+i nu L2}, e^{-i eta L3}, e^{+i eta L3}
+  double ph[4][2];
+  cyx::KbTables<double> kb_d;
+  cyx::KbTables<float> kb_f;
+  cyx::KbTables<cyx::cplx<double>> kb_z;
+  cyx::KbTables<cyx::cplx<float>> kb_c;
+  int nproj = 0;
+  // lazily grown scratch
+  cyx::DeviceBuf kbC;      // C coefficients (ncol x nproj x 16 B)
+  cyx::DeviceBuf small;    // Gram partial sums
+  cyx::DeviceBuf sub;      // M, Hs, eigenvalues, solver work/info, column scales
+  int64_t N() const { return (int64_t)g.nr * g.nt * g.nz; }
+};
+
+struct cyx_poisson_s {
+  cyx_ctx_s* ctx = nullptr;
+  cyx_grid_desc g{};
+  double* coef = nullptr;   // [nr][RCOEF] Lap_std coefficients (no -1/2)
+  double* diag = nullptr;   // [nr] diagonal of Lap_std per radius
+  double* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int64_t N() const { return (int64_t)g.nr * g.nt * g.nz; }
+};
+
+namespace cyx {
+
+int ensure(DeviceBuf& b, size_t bytes);
+
+// stencil step: out = alpha*((H - shift) in) - beta*prev   (prev may be null;
+// out may alias prev).  Dispatches on dtype.  Includes the KB term when the
+// handle has projectors (gather launched internally on `in`).
+int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void* out, int ncol, int64_t ld,
+                 double alpha, double shift, double beta);
+
+// subspace kernels (FP64 accumulation)
+int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out /*dev*/);
+int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
+int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* scal /*dev ns*/);
+int hermitize(cyx_ham_s* h, int dtype, void* M, int ns);
+
+// Poisson kernels
+int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
+                double omega, double beta, int m, int p, cyx_aar_info* info);
+int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f);
+
+// SCF glue
+int density_accumulate(cyx_ham_s* h, int dtype, const void* psi, int ns, const double* occ_dev, double c, double* rho);
+int lda_pw(hipStream_t st, int64_t N, const double* rho, const double* phi, double* vxc, double* veff, double rhotol);
+int energy_terms(hipStream_t st, int64_t N, const double* rho, const double* vxc, const double* phi, const double* b,
+                 const double* W, double rhotol, double* part_dev, int nblk, double out[3]);
+int veff_from_device(cyx_ham_s* h, const double* veff);
+
+size_t dtype_size(int dtype);
+
+}  // namespace cyx

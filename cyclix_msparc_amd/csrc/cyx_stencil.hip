@@ -1,0 +1,403 @@
+// Fused Cyclix Hamiltonian stencil + Chebyshev three-term update (gfx950).
+//
+// out = alpha * ((H - shift) in) - beta * prev, with
+//   H = -1/2 Lap + Veff + KB,
+//   Lap = (D_rr + r^-1 D_r) + (tau^2 + r^-2) D_tt + D_zz + (-2 tau D_t) (x) D_z
+// in (r, theta, z), radial Dirichlet, theta/z periodic with Bloch phases.
+// This is h_nonlocal_vector_mult (src/h_nonlocal_vector_mult.m:21-43) over
+// lapVec_cychel (Cychel/lapVec_cychel.m:19-53) folded into one step of
+// chebyshev_filter (src/chebyshev_filter.m:29-37).
+//
+// Decomposition: one workgroup owns an r-tile x the whole theta ring of one
+// column, and marches along z.  Each incoming z-plane is staged in LDS with
+// its r halo and a theta halo whose wrapped copies already carry the Bloch
+// phase; the z direction (and the z half of the helical cross term) is
+// accumulated in a 13-deep register queue of output planes whose indices are
+// compile-time constants (the plane loop is unrolled by 13).  An output plane
+// is complete FDN planes after it entered the queue; it then takes the KB
+// scatter (CSR by node), the recurrence, and is written in place over prev.
+#include "cyx_internal.hpp"
+
+namespace cyx {
+
+// workgroup size cap per element size: complex128 needs ~200 VGPRs per lane
+template <class T>
+constexpr int stencil_max_threads() {
+  return sizeof(T) >= 16 ? 512 : 768;
+}
+
+template <class T>
+struct StepParams {
+  using R = typename traits<T>::real;
+  const T* in;
+  const T* prev;
+  T* out;
+  int64_t ld;
+  int nr, nt, nz;
+  int tr, zb;
+  int ntiles_r, nblocks_z;
+  int xcd_remap;
+  R alpha, beta, shift;
+  const R* rcoef;
+  const R* veff;
+  R w2[FDN + 1], w1[FDN + 1];  // FD weights (src/initialization.m:403-419)
+  R s_r2, s_r1, s_z2, s_x;      // -1/2/dr^2, -1/2/dr, -1/2/dz^2, (-2 tau/dt)(-1/2/dz)
+  R xin, dr;
+  T ph_tl, ph_tr, ph_zl, ph_zr;
+  const int* s_ptr;
+  const int* s_proj;
+  const T* s_coef;
+  const T* kbC;
+  int nproj;
+};
+
+template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+__global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(const StepParams<T> P) {
+  using R = typename traits<T>::real;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+
+  const int TRH = P.tr + 2 * FDN;
+  const int plane_sz = TRH * (P.nt + 2 * FDN);
+
+  int bid = blockIdx.x;
+  if (P.xcd_remap) {
+    const int per = gridDim.x >> 3;
+    bid = (bid & 7) * per + (bid >> 3);
+  }
+  const int tile = bid % P.ntiles_r;
+  const int zblk = (bid / P.ntiles_r) % P.nblocks_z;
+  const int s = bid / (P.ntiles_r * P.nblocks_z);
+
+  const int tid = threadIdx.x;
+  const int i = tid % P.tr;
+  const int j = tid / P.tr;
+  const int r0 = tile * P.tr;
+  const int r = r0 + i;
+  const bool rvalid = r < P.nr;
+  const int z0 = zblk * P.zb;
+  const int z1 = min(z0 + P.zb, P.nz);
+  const int nrt = P.nr * P.nt;
+  const int64_t colbase = (int64_t)s * P.ld;
+  const T* __restrict__ src = P.in + colbase;
+
+  // per-radius scalars: 1/r, theta scale, diagonal (radial taps are
+  // cr2_p (x+ + x-) + (1/r) cr1_p (x+ - x-) with uniform cr2, cr1)
+  R rfac = 0, cth = 0, cdg = 0;
+  if (rvalid) {
+    rfac = P.s_r1 / (P.xin + R(r) * P.dr);
+    cth = P.rcoef[r * RCOEF + 2 * FDN];
+    cdg = P.rcoef[r * RCOEF + 2 * FDN + 1] - P.shift;
+  }
+
+  T q[QN];
+#pragma unroll
+  for (int u = 0; u < QN; ++u) q[u] = zero<T>();
+
+  const int kbeg = z0 - FDN;
+  const int nplanes = (z1 - z0) + 2 * FDN;
+
+  // plane loader: virtual plane kk -> registers (with z Bloch phase)
+  auto load_plane = [&](int kk, T& xa, T& xb, R& ve) {
+    int kp = kk;
+    int wrap = 0;
+    if (kp < 0) {
+      kp += P.nz;
+      wrap = -1;
+    } else if (kp >= P.nz) {
+      kp -= P.nz;
+      wrap = 1;
+    }
+    const T* row = src + (int64_t)kp * nrt + (int64_t)j * P.nr;
+    const int ra = r0 - FDN + i;
+    xa = (ra >= 0 && ra < P.nr) ? row[ra] : zero<T>();
+    xb = zero<T>();
+    if (i < 2 * FDN) {
+      const int rb = r0 - FDN + P.tr + i;
+      if (rb < P.nr) xb = row[rb];
+    }
+    if constexpr (traits<T>::is_complex) {
+      if (wrap != 0) {
+        const T ph = wrap < 0 ? P.ph_zl : P.ph_zr;
+        xa = cmul(ph, xa);
+        xb = cmul(ph, xb);
+      }
+    }
+    ve = (rvalid && kk >= z0 && kk < z1) ? P.veff[(int64_t)kp * nrt + (int64_t)j * P.nr + r] : R(0);
+  };
+
+  T xa, xb;
+  R ve;
+  load_plane(kbeg, xa, xb, ve);
+  int buf = 0;
+
+  for (int base = 0; base < nplanes; base += QN) {
+#pragma unroll
+    for (int u = 0; u < QN; ++u) {
+      const int t = base + u;
+      if (t < nplanes) {
+        const int kk = kbeg + t;
+        // ---- stage plane kk in LDS (theta halo carries the Bloch phase)
+        T* L = lds + buf * plane_sz;
+        {
+          const int row = (j + FDN) * TRH;
+          L[row + i] = xa;
+          if (i < 2 * FDN) L[row + P.tr + i] = xb;
+          if (j < FDN) {
+            const int rw = (j + FDN + P.nt) * TRH;
+            L[rw + i] = cmul(P.ph_tr, xa);
+            if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tr, xb);
+          }
+          if (j >= P.nt - FDN) {
+            const int rw = (j + FDN - P.nt) * TRH;
+            L[rw + i] = cmul(P.ph_tl, xa);
+            if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tl, xb);
+          }
+        }
+        const R vcur = ve;
+        __syncthreads();
+        // ---- prefetch: next plane, and prev of the plane emitted this step
+        if (t + 1 < nplanes) load_plane(kk + 1, xa, xb, ve);
+        const int ko = kk - FDN;
+        const bool emit = (ko >= z0) && (ko < z1);
+        const int64_t nout = (int64_t)ko * nrt + (int64_t)j * P.nr + r;
+        T pv = zero<T>();
+        if constexpr (HAS_PREV) {
+          if (emit && rvalid) pv = P.prev[colbase + nout];
+        }
+        // ---- compute plane kk
+        const int c = (j + FDN) * TRH + (i + FDN);
+        const T x0 = L[c];
+        T tt = zero<T>();
+        T st = zero<T>();
+#pragma unroll
+        for (int p = 1; p <= FDN; ++p) {
+          const T xp = L[c + p * TRH];
+          const T xm = L[c - p * TRH];
+          st = rfma(P.w2[p], add(xp, xm), st);
+          if constexpr (HELICAL) tt = rfma(P.w1[p], sub(xp, xm), tt);
+        }
+        if (kk >= z0 && kk < z1) {
+          T sr = zero<T>();
+          T dr = zero<T>();
+#pragma unroll
+          for (int p = 1; p <= FDN; ++p) {
+            const T xp = L[c + p];
+            const T xm = L[c - p];
+            sr = rfma(P.w2[p], add(xp, xm), sr);
+            dr = rfma(P.w1[p], sub(xp, xm), dr);
+          }
+          T self = rfma(P.s_r2, sr, rmul(cdg + vcur, x0));
+          self = rfma(rfac, dr, self);
+          self = rfma(cth, st, self);
+          q[u % QN] = add(q[u % QN], self);
+        }
+        {
+          const T xz = rmul(P.s_z2, x0);
+          T tz = zero<T>();
+          if constexpr (HELICAL) tz = rmul(P.s_x, tt);
+#pragma unroll
+          for (int d = 1; d <= FDN; ++d) {
+            // plane kk feeds output planes kk-d and kk+d
+            T& qa = q[(u - d + QN) % QN];
+            T& qb = q[(u + d) % QN];
+            qa = rfma(P.w2[d], xz, qa);
+            qb = rfma(P.w2[d], xz, qb);
+            if constexpr (HELICAL) {
+              qa = rfma(P.w1[d], tz, qa);
+              qb = rfma(-P.w1[d], tz, qb);
+            }
+          }
+        }
+        // ---- output plane kk-FDN is complete
+        {
+          T& qo = q[(u + FDN + 1) % QN];
+          if (emit && rvalid) {
+            T h = qo;
+            if constexpr (HAS_KB) {
+              const int e1 = P.s_ptr[nout + 1];
+              for (int e = P.s_ptr[nout]; e < e1; ++e)
+                h = cfma(P.s_coef[e], P.kbC[(int64_t)s * P.nproj + P.s_proj[e]], h);
+            }
+            T y = rmul(P.alpha, h);
+            if constexpr (HAS_PREV) y = rfma(-P.beta, pv, y);
+            P.out[colbase + nout] = y;
+          }
+          qo = zero<T>();
+        }
+        buf ^= 1;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- KB gather
+// C[s][p] = gamma_p * sum_e g[e] * Y[s][node[e]]   (src/h_nonlocal_vector_mult.m:30-38)
+template <class T>
+__global__ void __launch_bounds__(256) kb_gather_kernel(const T* __restrict__ Y, int64_t ld, const int* __restrict__ g_ptr,
+                                                        const int* __restrict__ g_node, const T* __restrict__ g_coef,
+                                                        const typename traits<T>::real* __restrict__ gamma, int nproj,
+                                                        T* __restrict__ C) {
+  __shared__ T red[256];
+  const int p = blockIdx.x;
+  const int s = blockIdx.y;
+  const T* y = Y + (int64_t)s * ld;
+  T acc = zero<T>();
+  const int e1 = g_ptr[p + 1];
+  for (int e = g_ptr[p] + threadIdx.x; e < e1; e += blockDim.x) acc = cfma(g_coef[e], y[g_node[e]], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) C[(int64_t)s * nproj + p] = rmul(gamma[p], red[0]);
+}
+
+// ---------------------------------------------------------------- launch
+template <class T>
+static const KbTables<T>& kb_of(cyx_ham_s* h);
+template <>
+const KbTables<double>& kb_of<double>(cyx_ham_s* h) {
+  return h->kb_d;
+}
+template <>
+const KbTables<float>& kb_of<float>(cyx_ham_s* h) {
+  return h->kb_f;
+}
+template <>
+const KbTables<cplx<double>>& kb_of<cplx<double>>(cyx_ham_s* h) {
+  return h->kb_z;
+}
+template <>
+const KbTables<cplx<float>>& kb_of<cplx<float>>(cyx_ham_s* h) {
+  return h->kb_c;
+}
+
+template <class R>
+static const OpCoef<R>& coef_of(cyx_ham_s* h);
+template <>
+const OpCoef<double>& coef_of<double>(cyx_ham_s* h) {
+  return h->c64;
+}
+template <>
+const OpCoef<float>& coef_of<float>(cyx_ham_s* h) {
+  return h->c32;
+}
+
+// r tile: largest tile with tr * nt <= maxthr and tr >= 2*FDN, balanced over nr
+static int choose_tr(int nr, int nt, int maxthr) {
+  int maxtr = maxthr / nt;
+  if (maxtr > nr) maxtr = nr;
+  if (maxtr < 2 * FDN) maxtr = 2 * FDN;
+  int ntiles = (nr + maxtr - 1) / maxtr;
+  int tr = (nr + ntiles - 1) / ntiles;
+  if (tr < 2 * FDN) tr = 2 * FDN;
+  return tr;
+}
+
+template <class T, bool HEL, bool PREV, bool KB>
+static void launch_variant(const StepParams<T>& P, int nblk, int nthr, size_t shm, hipStream_t st) {
+  hipLaunchKernelGGL((stencil_step_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
+}
+
+template <class T>
+static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out_, int ncol, int64_t ld, double alpha,
+                     double shift, double beta) {
+  using R = typename traits<T>::real;
+  const cyx_grid_desc& g = h->g;
+  const OpCoef<R>& C = coef_of<R>(h);
+  const KbTables<T>& K = kb_of<T>(h);
+  hipStream_t st = h->ctx->stream;
+  const int64_t N = h->N();
+  const bool has_kb = K.nproj > 0;
+  if (has_kb) {
+    CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
+    PhaseTimer tm(h->ctx, PH_KB_GATHER);
+    hipLaunchKernelGGL(kb_gather_kernel<T>, dim3(K.nproj, ncol), dim3(256), 0, st, (const T*)in_, ld, K.g_ptr,
+                       K.g_node, K.g_coef, K.gamma, K.nproj, (T*)h->kbC.p);
+    CYX_HIP(hipGetLastError());
+  }
+  StepParams<T> P{};
+  P.in = (const T*)in_;
+  P.prev = (const T*)prev_;
+  P.out = (T*)out_;
+  P.ld = ld;
+  P.nr = g.nr;
+  P.nt = g.nt;
+  P.nz = g.nz;
+  P.tr = choose_tr(g.nr, g.nt, stencil_max_threads<T>());
+  P.ntiles_r = ceil_div(g.nr, P.tr);
+  int zb = g.nz;
+  while ((long)P.ntiles_r * ceil_div(g.nz, zb) * ncol < 512 && zb > 4) zb = (zb + 1) / 2;
+  P.zb = zb;
+  P.nblocks_z = ceil_div(g.nz, zb);
+  P.alpha = (R)alpha;
+  P.beta = (R)beta;
+  P.shift = (R)shift;
+  P.rcoef = C.rcoef;
+  P.veff = C.veff;
+  for (int p = 0; p <= FDN; ++p) {
+    P.w2[p] = C.w2[p];
+    P.w1[p] = C.w1[p];
+  }
+  P.s_r2 = C.s_r2;
+  P.s_r1 = C.s_r1;
+  P.s_z2 = C.s_z2;
+  P.s_x = C.s_x;
+  P.xin = (R)g.xin;
+  P.dr = (R)g.dr;
+  P.ph_tl = phase_of<T>(h->ph[0][0], h->ph[0][1]);
+  P.ph_tr = phase_of<T>(h->ph[1][0], h->ph[1][1]);
+  P.ph_zl = phase_of<T>(h->ph[2][0], h->ph[2][1]);
+  P.ph_zr = phase_of<T>(h->ph[3][0], h->ph[3][1]);
+  P.s_ptr = K.s_ptr;
+  P.s_proj = K.s_proj;
+  P.s_coef = K.s_coef;
+  P.kbC = (const T*)h->kbC.p;
+  P.nproj = K.nproj;
+  const long nblk = (long)P.ntiles_r * P.nblocks_z * ncol;
+  CYX_CHECK(nblk < (1L << 31), "grid too large");
+  P.xcd_remap = (nblk % 8 == 0) ? 1 : 0;
+  const int nthr = P.tr * g.nt;
+  const size_t shm = 2 * (size_t)(P.tr + 2 * FDN) * (g.nt + 2 * FDN) * sizeof(T);
+  const bool hel = g.cell_typ == 4 || g.cell_typ == 5;
+  const bool prev = prev_ != nullptr;
+  (void)N;
+#define CYX_LAUNCH(HEL, PRV, KBV) launch_variant<T, HEL, PRV, KBV>(P, (int)nblk, nthr, shm, st)
+  PhaseTimer tm(h->ctx, prev ? PH_STENCIL : PH_HAPPLY);
+  if (hel) {
+    if (prev) {
+      if (has_kb) CYX_LAUNCH(true, true, true); else CYX_LAUNCH(true, true, false);
+    } else {
+      if (has_kb) CYX_LAUNCH(true, false, true); else CYX_LAUNCH(true, false, false);
+    }
+  } else {
+    if (prev) {
+      if (has_kb) CYX_LAUNCH(false, true, true); else CYX_LAUNCH(false, true, false);
+    } else {
+      if (has_kb) CYX_LAUNCH(false, false, true); else CYX_LAUNCH(false, false, false);
+    }
+  }
+#undef CYX_LAUNCH
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void* out, int ncol, int64_t ld,
+                 double alpha, double shift, double beta) {
+  switch (dtype) {
+    case CYX_F64:
+      return step_impl<double>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+    case CYX_F32:
+      return step_impl<float>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+    case CYX_C128:
+      return step_impl<cplx<double>>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+    case CYX_C64:
+      return step_impl<cplx<float>>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+  }
+  set_error("unknown dtype %d", dtype);
+  return CYX_ERR_ARG;
+}
+
+}  // namespace cyx

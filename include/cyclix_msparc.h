@@ -1,0 +1,192 @@
+/*
+ * cyclix_msparc.h -- C-ABI of the MI355X (gfx950) CheFSI + AAR-Poisson library.
+ *
+ * Drop-in boundary for the hot path of Cyclix-M-SPARC's SCF step.  A MATLAB
+ * host binds these symbols from a MEX gateway; every entry point names the
+ * reference routine it replaces (paths relative to Cyclix_MS_200220/).
+ *
+ * Conventions
+ *  - Grid functions are length-N vectors, N = nr*nt*nz, radial index fastest,
+ *    then theta, then z (src/calculate_nloc_projector.m:145).
+ *  - A block of states is column-major N x ncol with leading dimension ld >= N
+ *    (MATLAB's psi(:,:,ks), src/scf.m:82).  Complex data is interleaved
+ *    (re, im), the MATLAB R2018a interleaved-complex layout.
+ *  - Pointers documented "device" are HIP device pointers owned by the caller;
+ *    "host" pointers are ordinary host memory.  No torch types cross the ABI.
+ *  - Every function returns CYX_OK (0) or a negative status; cyx_last_error()
+ *    returns a message for the last failure on the calling thread.  No C++
+ *    exception crosses the ABI.
+ *  - All work is enqueued on the context's HIP stream; functions that return
+ *    host-visible results (eigenvalues, AAR info) synchronise that stream.
+ *  - A context (and every handle made from it) is single-caller: calls that
+ *    overlap on one handle race.
+ */
+#ifndef CYCLIX_MSPARC_H
+#define CYCLIX_MSPARC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CYX_OK 0
+#define CYX_ERR_ARG (-1)      /* invalid argument or shape */
+#define CYX_ERR_HIP (-2)      /* HIP runtime error */
+#define CYX_ERR_NOCONV (-3)   /* iterative solver hit max_iter (result still returned) */
+#define CYX_ERR_NOMEM (-4)    /* device allocation failed */
+#define CYX_ERR_SOLVER (-5)   /* dense solver failure (e.g. overlap not positive definite) */
+#define CYX_ERR_STATE (-6)    /* handle used in a wrong state */
+
+/* arithmetic types of a block of states */
+#define CYX_F64 0  /* real double      (Gamma channel, FP64)            */
+#define CYX_C128 1 /* complex double   (Bloch channel, FP64)            */
+#define CYX_F32 2  /* real float       (mixed precision, Gamma)         */
+#define CYX_C64 3  /* complex float    (mixed precision, Bloch channel) */
+
+/* One cyclic/helical sector.  Fields are the reference's S.Nx, S.Ny, S.Nz
+ * (node counts incl. the Dirichlet node, src/initialization.m:341-344),
+ * S.FDn, S.cell_typ (3 cyclic, 4/5 helical), S.xin, S.dx, S.dy, S.dz, S.twist.
+ * Radial direction Dirichlet, theta and z periodic with Bloch phases. */
+typedef struct cyx_grid_desc {
+  int32_t nr, nt, nz;
+  int32_t fdn;
+  int32_t cell_typ;
+  int32_t reserved;
+  double xin, dr, dt, dz;
+  double twist;
+} cyx_grid_desc;
+
+/* Kleinman-Bylander projectors: S.Atom(J).rcImage(img).{rc_pos,Chi_mat,
+ * coordinates} and S.Atom(J).gamma_Jl flattened over atoms and images
+ * (src/calculate_nloc_projector.m:185-277).  All host pointers. */
+typedef struct cyx_projector_desc {
+  int32_t n_atom;               /* S.n_atm                                            */
+  int32_t n_image;              /* rc-images summed over atoms                        */
+  const int32_t* angnum;        /* [n_atom] projectors per atom (S.Atom(J).angnum)     */
+  const double* gamma;          /* [sum angnum] gamma_Jl, atoms concatenated           */
+  const int32_t* image_atom;    /* [n_image] 0-based owning atom                       */
+  const int32_t* image_nnodes;  /* [n_image] rc_n_nodes                                */
+  const double* image_disp;     /* [3*n_image] S.Atoms(J,:) - rcImage(img).coordinates */
+  const int32_t* rc_pos;        /* [sum nnodes] 0-based node index, images concatenated */
+  const double* chi;            /* per image, Chi_mat column-major nnodes x angnum      */
+} cyx_projector_desc;
+
+typedef struct cyx_ctx_s* cyx_ctx_t;
+typedef struct cyx_ham_s* cyx_ham_t;
+typedef struct cyx_poisson_s* cyx_poisson_t;
+
+/* ------------------------------------------------------------ context */
+const char* cyx_last_error(void);
+const char* cyx_version(void);
+/* Creates a context on HIP device `device` with its own (default-priority)
+ * stream.  Replaces nothing in the reference (MATLAB owns no device). */
+int cyx_ctx_create(int device, cyx_ctx_t* out);
+int cyx_ctx_destroy(cyx_ctx_t ctx);
+/* Use a caller-owned HIP stream (hipStream_t) instead of the context's. */
+int cyx_ctx_set_stream(cyx_ctx_t ctx, void* hip_stream);
+int cyx_ctx_synchronize(cyx_ctx_t ctx);
+/* Per-phase device timers (HIP events recorded on the context's stream
+ * around every launch of a phase).  Phases: "stencil" (fused H-apply +
+ * three-term Chebyshev step), "h_apply" (stencil launches without the
+ * three-term update: first filter step, H psi), "kb_gather", "gram",
+ * "rotate", "wnorm", "dense_eig", "aar" (one whole AAR solve).  The reference prints only tic/toc wall
+ * times (src/scf.m:284-304, src/poissonSolve.m:49). */
+int cyx_ctx_set_timing(cyx_ctx_t ctx, int enable);
+int cyx_ctx_reset_timing(cyx_ctx_t ctx);
+int cyx_ctx_read_timing(cyx_ctx_t ctx, const char* phase, double* total_ms, int64_t* count);
+
+/* ------------------------------------------------------------ Hamiltonian */
+/* Device-resident operator of one Bloch channel: the 1-D factors of
+ * blochLaplacian_1d (src/blochLaplacian_1d.m:1-123 over
+ * Cychel/lapIndicesValues_1d_cychel.m:9-261), S.W, S.Veff and the KB tables.
+ * kpt = S.kptgrid(kpt,:) = [0, nu, eta].  veff, W: host, length N. */
+int cyx_ham_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, const double kpt[3], const double* veff,
+                   const double* W, const cyx_projector_desc* proj, cyx_ham_t* out);
+/* Replace Veff (S.Veff(:,spin)); `on_device` selects whether veff is a device pointer. */
+int cyx_ham_set_veff(cyx_ham_t ham, const double* veff, int on_device);
+int cyx_ham_destroy(cyx_ham_t ham);
+/* Device pointer of the FP64 weights W held by the handle (length N). */
+const double* cyx_ham_weights(cyx_ham_t ham);
+
+/* HX = H X.  Replaces h_nonlocal_vector_mult (src/h_nonlocal_vector_mult.m:1-43).
+ * X, HX device, column-major N x ncol, leading dimension ld. */
+int cyx_h_apply(cyx_ham_t ham, int dtype, const void* X, void* HX, int ncol, int64_t ld);
+
+/* Y = p_m(H) X, Zhou-Saad scaled Chebyshev filter of degree m on [a, b] with
+ * lower bound a0.  Replaces chebyshev_filter (src/chebyshev_filter.m:1-42).
+ * X and Y device N x ncol (leading dimension ld); X is overwritten (used as
+ * the recurrence's second buffer). */
+int cyx_chebyshev_filter(cyx_ham_t ham, int dtype, void* X, void* Y, int ncol, int64_t ld, int m, double a,
+                         double b, double a0);
+
+/* Bytes of device workspace cyx_eigsolver_pass needs for `ns` states. */
+int64_t cyx_eigsolver_workspace_bytes(cyx_ham_t ham, int dtype, int ns);
+
+/* One CheFSI pass of one channel: the body of eigSolver's channel loop
+ * (src/eigSolver.m:98-124): filter of degree npl on [lambda_cutoff, bup] with
+ * lower bound a0, then W-metric Rayleigh-Ritz (overlap Cholesky and projected
+ * Hermitian eigenproblem in FP64 on the device), rotation and W-normalisation.
+ * psi: device N x ns (ld = N), in/out.  work: device, at least
+ * cyx_eigsolver_workspace_bytes.  eigval: host, ns doubles, ascending. */
+int cyx_eigsolver_pass(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, int npl, double lambda_cutoff,
+                       double bup, double a0, double* eigval);
+
+/* Rayleigh-Ritz only (src/eigSolver.m:99-124) on an already filtered psi. */
+int cyx_rayleigh_ritz(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, double* eigval);
+
+/* Extreme Ritz values of H by Lanczos from start vector v0 (host, N doubles,
+ * real part used); stands in for eigs(Hfun,N,1,'lr'/'sr') at
+ * src/eigSolver.m:47-63 (tolerance TOL_LANCZOS, at most maxit steps). */
+int cyx_lanczos_bounds(cyx_ham_t ham, const double* v0, int maxit, double tol, double* lambda_min,
+                       double* lambda_max);
+
+/* ------------------------------------------------------------ SCF glue (FP64) */
+/* rho += 2 * weight * sum_s occ[s] |psi_s|^2 over the ns device states psi
+ * (ld = N).  Replaces one k-point term of electronDensity
+ * (src/electronDensity.m:13-17).  occ: host, ns doubles; rho: device. */
+int cyx_density_accumulate(cyx_ham_t ham, int dtype, const void* psi, int ns, const double* occ, double weight,
+                           double* rho);
+/* vxc = LDA Perdew-Wang potential of rho with rho clamped at rhotol
+ * (src/exchangeCorrelationPotential.m:82-106); if veff is non-null also
+ * veff = phi + vxc (src/scf.m:39).  All device, length N. */
+int cyx_lda_pw(cyx_ctx_t ctx, int64_t N, const double* rho, const double* phi, double* vxc, double* veff,
+               double rhotol);
+/* out = {Exc, Exc_dc, Eelec_dc}: the grid sums of evaluateTotalEnergy
+ * (src/evaluateTotalEnergy.m:27-52,72) with the handle's weights S.W. */
+int cyx_energy_terms(cyx_ham_t ham, const double* rho, const double* vxc, const double* phi, const double* b,
+                     double rhotol, double out[3]);
+
+/* ------------------------------------------------------------ Poisson */
+typedef struct cyx_aar_info {
+  int32_t iterations; /* count-1 at exit, as src/aar.m:394-397 reports */
+  int32_t converged;  /* relres <= tol */
+  double relres;      /* ||f - A x|| / ||f|| of the returned x */
+} cyx_aar_info;
+
+/* Device operator for the Poisson problem: S.Lap_std at k = 0
+ * (src/initialization.m:734-741) with a Jacobi preconditioner (the diagonal
+ * of Lap_std; extras/AAR_old.m:45-46,73-75). */
+int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* out);
+int cyx_poisson_destroy(cyx_poisson_t ps);
+
+/* x = aar(Lap_std, f, x_guess, tol, max_iter, omega, beta, m, p, Jacobi).
+ * Replaces aar (src/aar.m:1-72).  f, x device FP64 length N; x holds the guess
+ * on entry (pass use_guess = 0 for the reference's ones(N,1) start) and the
+ * solution on exit.  Returns CYX_ERR_NOCONV (with x and info filled) when
+ * max_iter is reached. */
+int cyx_aar(cyx_poisson_t ps, const double* f, double* x, int use_guess, double tol, int max_iter, double omega,
+            double beta, int m, int p, cyx_aar_info* info);
+
+/* S.phi = poissonSolve(S, tol, Isguess) for Cyclix (src/poissonSolve.m:17-51):
+ * f = -4 pi (rho + b), then cyx_aar with omega = beta = 0.6, m = 7, p = 6.
+ * rho, b, phi device FP64 length N. */
+int cyx_poisson_solve(cyx_poisson_t ps, const double* rho, const double* b, double* phi, int is_guess, double tol,
+                      int max_iter, cyx_aar_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CYCLIX_MSPARC_H */

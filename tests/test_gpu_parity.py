@@ -1,0 +1,188 @@
+"""HIP path vs the FP64 CPU oracle, through the C-ABI.
+
+Small grids (N of a few thousand) run the oracle in seconds; every dtype and
+both symmetries (cyclic Gamma, helical Bloch channel) are covered, with r
+tiles and z blocks split so that tile seams and z wraps are exercised.
+Tolerances: FP64 paths 1e-11 relative to the block's max; FP32 paths 2e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import cyclix_oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (nr, nt, nz, twist, kpt, dtypes)
+    "cyclic_gamma": (50, 16, 14, 0.0, (0.0, 0.0, 0.0), ("f64", "f32", "c128", "c64")),
+    "helical_bloch": (45, 16, 14, 0.05, (0.0, 1.0, 0.6), ("c128", "c64")),
+    "cyclic_bloch": (20, 13, 12, 0.0, (0.0, 2.0, 0.0), ("c128",)),
+}
+TOL = {"f64": 1e-11, "c128": 1e-11, "f32": 2e-5, "c64": 2e-5}
+NP = {"f64": np.float64, "c128": np.complex128, "f32": np.float32, "c64": np.complex64}
+
+
+def make_case(name):
+    from cyclix_msparc_amd.system import small_system
+    nr, nt, nz, tw, kpt, dts = CASES[name]
+    sysm = small_system(nr=nr, nt=nt, nz=nz, twist=tw, kpt=kpt, nstates=8, npl=10)
+    g = sysm.grid
+    r = np.tile(g.r, g.nt * g.nz)
+    R = sysm.atoms[:, 0].max()
+    veff = -1.5 * np.exp(-((r - R) / 1.2) ** 2) + 0.1 * np.cos(np.arange(g.N) * 0.37)
+    proj = sysm.projectors()
+    og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
+    atoms = [O.ProjAtom(a.coords, a.gamma, [O.ProjImage(im.rc_pos, im.chi, im.coords) for im in a.images])
+             for a in proj]
+    oham = O.Hamiltonian(og, tuple(kpt), veff, sysm.W, atoms)
+    return sysm, veff, proj, oham, dts
+
+
+def to_dev(X, dt, dev):
+    """(N, ns) column-major numpy block -> (ns, N) device tensor."""
+    return torch.from_numpy(np.ascontiguousarray(X.T.astype(NP[dt]))).to(dev.torch_device)
+
+
+def to_host(T):
+    return T.cpu().numpy().T.astype(np.complex128 if T.is_complex() else np.float64)
+
+
+def rand_block(N, ns, dt, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((N, ns))
+    if dt in ("c128", "c64"):
+        X = X + 1j * rng.standard_normal((N, ns))
+    return X
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_h_apply_parity(device, name):
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    for dt in dts:
+        X = rand_block(sysm.grid.N, 5, dt, 11)
+        Xq = X.astype(NP[dt]).astype(X.dtype)  # the block the device sees
+        HX = to_host(ham.apply(to_dev(X, dt, device)))
+        ref = oham.apply(Xq)
+        print(f"h_apply {name} {dt}: rel err {rel(HX, ref):.3e}", flush=True)
+        assert rel(HX, ref) < TOL[dt], (name, dt)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_chebyshev_filter_parity(device, name):
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    lo, hi = O.lanczos_bounds(oham.apply, np.random.default_rng(1).random(sysm.grid.N), 300, 1e-3,
+                              O.metric_weights(oham.g))
+    bup, a0 = 1.01 * hi, lo - 0.1
+    cut = 0.5 * (bup + a0)
+    for dt in dts:
+        for m in (1, 2, 7):
+            X = rand_block(sysm.grid.N, 4, dt, 5)
+            Xq = X.astype(NP[dt]).astype(X.dtype)
+            Y = to_host(ham.chebyshev_filter(to_dev(X, dt, device), m, cut, bup, a0))
+            ref = O.chebyshev_filter(oham.apply, Xq, m, cut, bup, a0)
+            tol = TOL[dt] * (10 if dt in ("f32", "c64") else 1) * m
+            print(f"filter {name} {dt} m={m}: rel err {rel(Y, ref):.3e}", flush=True)
+            assert rel(Y, ref) < tol, (name, dt, m)
+
+
+def _subspace_projector(psi, Wm, V):
+    M = psi.conj().T @ (Wm[:, None] * psi)
+    return psi @ np.linalg.solve(M, psi.conj().T @ (Wm[:, None] * V))
+
+
+@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch"])
+def test_eigsolver_pass_parity(device, name):
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    N, ns, npl = sysm.grid.N, 8, 10
+    Wm = O.metric_weights(oham.g)
+    lo, hi = O.lanczos_bounds(oham.apply, np.random.default_rng(1).random(N), 300, 1e-3, Wm)
+    bup, a0 = 1.01 * hi, lo - 0.1
+    cut = 0.5 * (bup + a0)
+    dt = "c128" if any(sysm.kpt) else "f64"
+    X = rand_block(N, ns, dt, 0)
+    ev_ref, psi_ref = O.eig_solver_pass(oham, X, npl, cut, bup, a0)
+    psi_d = to_dev(X, dt, device)
+    ev = ham.eig_pass(psi_d, npl, cut, bup, a0)
+    psi = to_host(psi_d)
+    print(f"eig pass {name}: eig err {np.max(np.abs(ev - ev_ref)):.3e}", flush=True)
+    assert np.allclose(ev, ev_ref, rtol=1e-9, atol=1e-9)
+    V = np.random.default_rng(3).standard_normal((N, 3))
+    assert rel(_subspace_projector(psi, Wm, V), _subspace_projector(psi_ref, Wm, V)) < 1e-8
+    # W-normalised columns (src/eigSolver.m:122-124)
+    nrm = np.sum(sysm.W[:, None] * np.abs(psi) ** 2, axis=0)
+    assert np.allclose(nrm, 1.0, atol=1e-12)
+    # several passes converge to the dense lowest eigenvalues
+    lam = np.sort(np.real(np.linalg.eigvals(oham.dense())))[:ns]
+    for it in range(30):
+        ev = ham.eig_pass(psi_d, npl, ev.max() + 0.1, bup, ev.min())
+    assert np.allclose(ev[:ns - 2], lam[:ns - 2], atol=1e-7)
+
+
+@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch"])
+def test_lanczos_bounds_parity(device, name):
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    v0 = np.random.default_rng(4).random(sysm.grid.N)
+    lo_ref, hi_ref = O.lanczos_bounds(oham.apply, v0.astype(complex) if any(sysm.kpt) else v0, 300, 1e-3,
+                                      O.metric_weights(oham.g))
+    lo, hi = ham.lanczos_bounds(v0, 300, 1e-3)
+    assert lo == pytest.approx(lo_ref, rel=1e-6, abs=1e-6)
+    assert hi == pytest.approx(hi_ref, rel=1e-6)
+
+
+@pytest.mark.parametrize("helical", [False, True])
+def test_aar_poisson_parity(device, helical):
+    from cyclix_msparc_amd.api import Poisson
+    from cyclix_msparc_amd.system import GridSpec
+    g = GridSpec(30, 16, 14, 2.0, 0.25, 2 * math.pi / 6 / 16, 0.35, 0.06 if helical else 0.0, 6,
+                 5 if helical else 3)
+    og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
+    rng = np.random.default_rng(8)
+    rho = np.abs(rng.standard_normal(g.N))
+    b = -rho.mean() * np.ones(g.N)
+    x_ref, relres_ref, it_ref = O.poisson_solve(og, rho, b, 1e-8, None, 1000, "jacobi")
+    ps = Poisson(device, g)
+    dev = device.torch_device
+    phi, info = ps.solve(torch.from_numpy(rho).to(dev), torch.from_numpy(b).to(dev), None, 1e-8, 1000)
+    phi = phi.cpu().numpy()
+    print(f"aar helical={helical}: it {info.iterations} vs {it_ref}, relres {info.relres:.2e}", flush=True)
+    assert info.converged and info.relres <= 1e-8
+    assert abs(info.iterations - it_ref) <= 2
+    assert np.linalg.norm(phi - x_ref) / np.linalg.norm(x_ref) < 1e-7
+    f = O.poisson_rhs(rho, b)
+    assert np.linalg.norm(f - O.lap_matrix(og) @ phi) / np.linalg.norm(f) <= 1.0001e-8
+
+
+def test_aar_reports_nonconvergence(device):
+    from cyclix_msparc_amd.api import Poisson
+    from cyclix_msparc_amd.system import GridSpec
+    g = GridSpec(20, 16, 14, 2.0, 0.25, 2 * math.pi / 6 / 16, 0.35, 0.0, 6, 3)
+    rho = torch.ones(g.N, dtype=torch.float64, device=device.torch_device)
+    b = -0.5 * rho
+    phi, info = Poisson(device, g).solve(rho, b, None, 1e-12, 3)
+    assert not info.converged and info.iterations == 3
+
+
+def test_errors_are_reported(device):
+    from cyclix_msparc_amd import _native as nat
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case("helical_bloch")
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    X = torch.zeros((2, sysm.grid.N), dtype=torch.float64, device=device.torch_device)
+    with pytest.raises(nat.CyxError):  # real dtype on a Bloch channel
+        ham.apply(X)
