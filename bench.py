@@ -41,7 +41,7 @@ def parse():
     p.add_argument("--workload", default="cnt10_helical")
     p.add_argument("--dtype", default="c128", choices=["c128", "c64"])
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample-cols", type=int, default=1)
+    p.add_argument("--cpu-sample-cols", type=int, default=16)
     return p.parse_args()
 
 
@@ -177,21 +177,23 @@ def main():
 
 
 def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args, tm):
-    """The FP64 CPU oracle on a bounded sample of the same SCF step, scaled
-    to one step: the filter on `cpu-sample-cols` columns (all npl degrees),
-    the Rayleigh-Ritz GEMMs at N/16 rows, and 20 AAR iterations; each part is
-    scaled to the full step (columns, rows, GPU iteration count)."""
+    """The plain-C FP64 oracle (OpenMP) on a bounded sample of the same SCF
+    step, scaled to one step: the degree-npl filter on `cpu-sample-cols`
+    columns (scaled by ns / cols), the three Rayleigh-Ritz GEMMs with numpy
+    at N/16 rows (scaled by 16) plus one H psi, and 20 Jacobi-AAR iterations
+    (scaled to the GPU's iteration count)."""
+    from oracle import c_oracle
     from oracle import cyclix_oracle as O
     g = sysm.grid
     N, ns, npl = g.N, sysm.nstates, sysm.npl
     og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
     atoms = [O.ProjAtom(a.coords, a.gamma, [O.ProjImage(i.rc_pos, i.chi, i.coords) for i in a.images]) for a in proj]
-    H = O.Hamiltonian(og, tuple(sysm.kpt), veff, sysm.W, atoms)
+    Cc = c_oracle.COracle(og, sysm.kpt, veff, sysm.W, atoms)
     c = args.cpu_sample_cols
     rng = np.random.default_rng(0)
     X = rng.standard_normal((N, c)) + 1j * rng.standard_normal((N, c))
     t = time.perf_counter()
-    O.chebyshev_filter(H.apply, X, npl, state["cut"], 1.0e3, state["a0"])
+    Cc.chebyshev_filter(X, npl, state["cut"], 1.0e3, state["a0"])
     t_filter = (time.perf_counter() - t) * ns / c
     n_rr = N // 16
     A = rng.standard_normal((n_rr, ns)) + 1j * rng.standard_normal((n_rr, ns))
@@ -200,17 +202,15 @@ def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args, tm):
     _ = A.conj().T @ A
     _ = A @ M
     t_rr = (time.perf_counter() - t) * 16 + t_filter / npl  # + one H psi
-    L = O.lap_matrix(og)
     f = O.poisson_rhs(pc["rho_at"], pc["b"])
-    d = L.diagonal()
     t = time.perf_counter()
-    O.aar(lambda v: L @ v, f, None, 1e-30, 20, 0.6, 0.6, 7, 6, lambda v: v / d)
+    Cc.aar_jacobi(f, None, 1e-30, 20)
     t_aar = (time.perf_counter() - t) / 20 * max(int(state["info"].iterations), 1)
     total = t_filter + t_rr + t_aar
-    cores = len(os.sched_getaffinity(0))
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": round(total * 1e3, 1), "unit": "ms/SCF-step", "cores": cores, "kind": "port",
-            "sample": f"numpy FP64 oracle: filter on {c} of {ns} columns (degree {npl}), RR GEMMs at N/16, "
-                      f"20 AAR iterations; each scaled to the full step "
+            "sample": f"C FP64 oracle (OpenMP, {cores} threads): filter on {c} of {ns} columns (degree {npl}), "
+                      f"numpy RR GEMMs at N/16, 20 Jacobi-AAR iterations; each scaled to the full step "
                       f"(filter {t_filter:.1f} s, RR {t_rr:.1f} s, AAR {t_aar:.1f} s)",
             "gpu_speedup": round(total * 1e3 / ms_step, 1)}
 

@@ -125,11 +125,39 @@ def test_eigsolver_pass_parity(device, name):
     # W-normalised columns (src/eigSolver.m:122-124)
     nrm = np.sum(sysm.W[:, None] * np.abs(psi) ** 2, axis=0)
     assert np.allclose(nrm, 1.0, atol=1e-12)
-    # several passes converge to the dense lowest eigenvalues
-    lam = np.sort(np.real(np.linalg.eigvals(oham.dense())))[:ns]
-    for it in range(30):
-        ev = ham.eig_pass(psi_d, npl, ev.max() + 0.1, bup, ev.min())
-    assert np.allclose(ev[:ns - 2], lam[:ns - 2], atol=1e-7)
+    # a few more passes: still the same iterates as the oracle
+    for it in range(4):
+        cut, a0 = ev.max() + 0.1, ev.min()
+        ev = ham.eig_pass(psi_d, npl, cut, bup, a0)
+        ev_ref, psi_ref = O.eig_solver_pass(oham, psi_ref, npl, cut, bup, a0)
+    assert np.allclose(ev, ev_ref, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("kpt,twist", [((0, 0, 0), 0.0), ((0, 1.0, 0.6), 0.05)])
+def test_chefsi_converges_to_dense_eigenvalues(device, kpt, twist):
+    """Repeated device passes reach the lowest eigenvalues of the dense H."""
+    from cyclix_msparc_amd.api import Hamiltonian
+    from cyclix_msparc_amd.system import small_system
+    sysm = small_system(nr=12, nt=14, nz=13, twist=twist, kpt=kpt, nstates=6)
+    g = sysm.grid
+    r = np.tile(g.r, g.nt * g.nz)
+    veff = -1.5 * np.exp(-((r - sysm.atoms[:, 0].max()) / 1.2) ** 2)
+    proj = sysm.projectors()
+    og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
+    oham = O.Hamiltonian(og, tuple(kpt), veff, sysm.W,
+                         [O.ProjAtom(a.coords, a.gamma, [O.ProjImage(i.rc_pos, i.chi, i.coords) for i in a.images])
+                          for a in proj])
+    lam = np.sort(np.real(np.linalg.eigvals(oham.dense())))
+    ham = Hamiltonian(device, sysm, kpt, veff, proj)
+    dt = "c128" if any(kpt) else "f64"
+    psi_d = to_dev(rand_block(g.N, 6, dt, 0), dt, device)
+    lo, hi = ham.lanczos_bounds(np.random.default_rng(0).random(g.N), 300, 1e-3)
+    bup, a0 = 1.01 * hi, lo - 0.1
+    cut = 0.5 * (bup + a0)
+    for it in range(40):
+        ev = ham.eig_pass(psi_d, 14, cut, bup, a0)
+        a0, cut = ev.min(), ev.max() + 0.1
+    assert np.allclose(ev, lam[:6], atol=1e-8)
 
 
 @pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch"])
