@@ -42,6 +42,10 @@ def parse():
     p.add_argument("--dtype", default="c128", choices=["c128", "c64"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
+    p.add_argument("--aar-maxit", type=int, default=1000)
+    p.add_argument("--calib-copy", action="store_true",
+                   help="after the timed region, run one elementwise copy of 512 MiB (read) + 512 MiB (write) "
+                        "as the byte-counter calibration kernel")
     return p.parse_args()
 
 
@@ -102,7 +106,7 @@ def main():
         lf, occ = fermi_occ(ev, nelec, bet)
         rho.zero_()
         ham.density_accumulate(rho, psi, occ, 1.0)
-        phi_new, info = poisson.solve(rho, b, state["phi"], 1e-8, 1000)
+        phi_new, info = poisson.solve(rho, b, state["phi"], 1e-8, args.aar_maxit)
         vxc_new, veff_new = lda_pw(dev, rho, phi_new)
         terms = ham.energy_terms(rho, vxc_new, phi_new, b)
         ham.set_veff(veff_new)
@@ -140,6 +144,12 @@ def main():
                     tm["kb_gather"][0] / args.steps * npl / (npl + 1))
     b_filter = (3 * npl - 1) * N * ns * bsz + npl * N * 8.0
     filter_gbs = b_filter / (filt_ms_only * 1e-3) / 1e9
+
+    if args.calib_copy:
+        src = torch.ones(1 << 26, dtype=torch.float64, device=tdev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu_baseline:
