@@ -20,6 +20,7 @@ DTYPE_NAMES = {"f64": CYX_F64, "c128": CYX_C128, "f32": CYX_F32, "c64": CYX_C64}
 EXPORTS = (
     "cyx_last_error", "cyx_version", "cyx_ctx_create", "cyx_ctx_destroy", "cyx_ctx_set_stream",
     "cyx_ctx_synchronize", "cyx_ctx_set_timing", "cyx_ctx_reset_timing", "cyx_ctx_read_timing", "cyx_ham_create", "cyx_ham_set_veff", "cyx_ham_destroy", "cyx_ham_weights",
+    "cyx_ham_set_tuning",
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_eigsolver_workspace_bytes", "cyx_eigsolver_pass",
     "cyx_rayleigh_ritz", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
@@ -46,6 +47,10 @@ class ProjectorDesc(C.Structure):
                 ("image_atom", C.POINTER(C.c_int32)), ("image_nnodes", C.POINTER(C.c_int32)),
                 ("image_disp", C.POINTER(C.c_double)), ("rc_pos", C.POINTER(C.c_int32)),
                 ("chi", C.POINTER(C.c_double))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tr", C.c_int32), ("zb", C.c_int32), ("flags", C.c_int32), ("reserved", C.c_int32)]
 
 
 class AarInfo(C.Structure):
@@ -79,6 +84,7 @@ def load():
         "cyx_ham_set_veff": (i32, [vp, vp, i32]),
         "cyx_ham_destroy": (i32, [vp]),
         "cyx_ham_weights": (vp, [vp]),
+        "cyx_ham_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "cyx_h_apply": (i32, [vp, i32, vp, vp, i32, i64]),
         "cyx_chebyshev_filter": (i32, [vp, i32, vp, vp, i32, i64, i32, dbl, dbl, dbl]),
         "cyx_eigsolver_workspace_bytes": (i64, [vp, i32, i32]),

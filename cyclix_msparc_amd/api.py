@@ -166,6 +166,11 @@ class Hamiltonian:
             v = np.ascontiguousarray(np.asarray(veff, dtype=float))
             nat.check(self.lib.cyx_ham_set_veff(self.h, v.ctypes.data_as(C.c_void_p), 0))
 
+    def set_tuning(self, tr=0, zb=0, nontemporal=False, no_xcd_remap=False):
+        """Launch shape of the fused stencil (results do not depend on it)."""
+        t = nat.Tuning(int(tr), int(zb), (1 if nontemporal else 0) | (2 if no_xcd_remap else 0), 0)
+        nat.check(self.lib.cyx_ham_set_tuning(self.h, C.byref(t)))
+
     def new_block(self, ns, dtype_code):
         return torch.empty((ns, self.N), dtype=TORCH_DTYPES[dtype_code], device=self.dev.torch_device)
 

@@ -3,19 +3,24 @@
 // extras/AAR_old.m:45-46,73-75).  Operator: S.Lap_std at k = 0
 // (src/initialization.m:734-741), applied matrix-free.
 //
-// The iteration runs without host round trips: every kernel of iteration
-// `count` first reads a device-side `done` flag and returns at once when an
-// earlier iteration already met the tolerance.  Reductions use per-block
-// partials and a last-block-done finalisation, summed in a fixed order, so a
-// run is bitwise reproducible.  The host enqueues iterations in chunks and
-// polls the flag between chunks.
+// One AAR iteration is two launches (three on Anderson steps):
+//   1. theta_kernel: T = (-2 tau D_theta) x  (helical cross term, separable)
+//   2. step kernel:  res1 = f - Lap x, relres partials, res = res1 ./ diag,
+//      history columns, and on Richardson steps x_new = x + omega res; the
+//      last block to finish sums the partials in a fixed order, decides
+//      convergence and, on Anderson steps, forms DF'DF, DF'res and solves
+//      gamma = pinv(DF'DF) DF'res;
+//   3. (Anderson) x_new = x + beta res - (DX + beta DF) gamma.
+// Kernels of an iteration that starts after convergence return at once, so
+// the host enqueues iterations in chunks and polls a device flag between
+// chunks.  Every reduction has a fixed order: a solve is bitwise reproducible.
 #include "cyx_internal.hpp"
 
 namespace cyx {
 
 struct AarState {
-  int done;       // 1 once relres <= tol
-  int iters;      // count at exit (as src/aar.m reports count-1 after the loop)
+  int done;       // 1 converged, 2 hit max_iter
+  int iters;      // count at exit (src/aar.m:394-397 reports count-1 after the loop)
   int final_buf;  // which x buffer holds the returned iterate
   int counter;    // last-block-done ticket
   double relres;  // relres of the returned iterate
@@ -28,68 +33,87 @@ constexpr int MAXHIST = 15;
 
 struct PoissonOp {
   int nr, nt, nz;
-  const double* coef;  // [nr][RCOEF]: c+[6], c-[6], c_theta, diag
-  double wt2[FDN + 1];  // w2_p (theta second derivative, times c_theta per radius)
-  double wz2[FDN + 1];  // w2_q / dz^2
-  double cx[FDN + 1][FDN + 1];  // (-2 tau w1_p / dt) (w1_q / dz)
   int helical;
+  const double* coef;   // [nr][RCOEF]: c+[6], c-[6], c_theta, diag (Lap_std, no -1/2)
+  double w2[FDN + 1];   // w2_p (theta second derivative, times c_theta per radius)
+  double wz2[FDN + 1];  // w2_q / dz^2
+  double wt1[FDN + 1];  // -2 tau w1_p / dt
+  double wz1[FDN + 1];  // w1_q / dz
 };
 
-__device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x, int i, int j, int k) {
+__device__ inline int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Launch shape of the per-point AAR kernels: 64 radial x 4 theta threads per
+// block, grid (ceil(nr/64), ceil(nt/4), nz); 32-bit index arithmetic.
+constexpr int TX = 64, TY = 4;
+
+struct Pt {
+  int i, j, k;
+  bool ok;
+  int n;
+};
+__device__ inline Pt point_of(const int nr, const int nt) {
+  Pt p;
+  p.i = blockIdx.x * TX + threadIdx.x;
+  p.j = blockIdx.y * TY + threadIdx.y;
+  p.k = blockIdx.z;
+  p.ok = p.i < nr && p.j < nt;
+  p.n = (p.k * nt + p.j) * nr + p.i;
+  return p;
+}
+
+// T[n] = sum_p wt1_p (x(j+p) - x(j-p))
+__global__ void __launch_bounds__(AB) theta_kernel(PoissonOp P, const double* __restrict__ x, double* __restrict__ T,
+                                                   const AarState* S) {
+  if (S->done) return;
+  const Pt q = point_of(P.nr, P.nt);
+  if (!q.ok) return;
+  const int base = q.n - q.j * P.nr;  // (i, 0, k)
+  double t = 0;
+#pragma unroll
+  for (int p = 1; p <= FDN; ++p)
+    t = __builtin_fma(P.wt1[p], x[base + wrapi(q.j + p, P.nt) * P.nr] - x[base + wrapi(q.j - p, P.nt) * P.nr], t);
+  T[q.n] = t;
+}
+
+__device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x, const double* __restrict__ T, int i,
+                                int j, int k, int n) {
   const int nrt = P.nr * P.nt;
   const double* cf = P.coef + i * RCOEF;
-  const int64_t n = (int64_t)k * nrt + j * P.nr + i;
   double acc = cf[2 * FDN + 1] * x[n];
 #pragma unroll
   for (int p = 1; p <= FDN; ++p) {
     if (i + p < P.nr) acc = __builtin_fma(cf[p - 1], x[n + p], acc);
     if (i - p >= 0) acc = __builtin_fma(cf[FDN + p - 1], x[n - p], acc);
   }
+  const int rowk = k * nrt + i;
   double st = 0.0;
 #pragma unroll
-  for (int p = 1; p <= FDN; ++p) {
-    int jp = j + p;
-    if (jp >= P.nt) jp -= P.nt;
-    int jm = j - p;
-    if (jm < 0) jm += P.nt;
-    st = __builtin_fma(P.wt2[p], x[(int64_t)k * nrt + jp * P.nr + i] + x[(int64_t)k * nrt + jm * P.nr + i], st);
-  }
+  for (int p = 1; p <= FDN; ++p)
+    st = __builtin_fma(P.w2[p], x[rowk + wrapi(j + p, P.nt) * P.nr] + x[rowk + wrapi(j - p, P.nt) * P.nr], st);
   acc = __builtin_fma(cf[2 * FDN], st, acc);
+  const int colj = j * P.nr + i;
 #pragma unroll
   for (int q = 1; q <= FDN; ++q) {
-    int kp = k + q;
-    if (kp >= P.nz) kp -= P.nz;
-    int km = k - q;
-    if (km < 0) km += P.nz;
-    acc = __builtin_fma(P.wz2[q], x[(int64_t)kp * nrt + j * P.nr + i] + x[(int64_t)km * nrt + j * P.nr + i], acc);
-  }
-  if (P.helical) {
-#pragma unroll
-    for (int q = 1; q <= FDN; ++q) {
-      int kp = k + q;
-      if (kp >= P.nz) kp -= P.nz;
-      int km = k - q;
-      if (km < 0) km += P.nz;
-#pragma unroll
-      for (int p = 1; p <= FDN; ++p) {
-        int jp = j + p;
-        if (jp >= P.nt) jp -= P.nt;
-        int jm = j - p;
-        if (jm < 0) jm += P.nt;
-        const double d = (x[(int64_t)kp * nrt + jp * P.nr + i] - x[(int64_t)kp * nrt + jm * P.nr + i]) -
-                         (x[(int64_t)km * nrt + jp * P.nr + i] - x[(int64_t)km * nrt + jm * P.nr + i]);
-        acc = __builtin_fma(P.cx[p][q], d, acc);
-      }
-    }
+    const int kp = wrapi(k + q, P.nz) * nrt + colj;
+    const int km = wrapi(k - q, P.nz) * nrt + colj;
+    acc = __builtin_fma(P.wz2[q], x[kp] + x[km], acc);
+    if (P.helical) acc = __builtin_fma(P.wz1[q], T[kp] - T[km], acc);
   }
   return acc;
 }
 
+// sum of v over the block (AB threads), result valid in every thread
+__device__ inline int tlin() { return threadIdx.x + blockDim.x * threadIdx.y; }
+__device__ inline int nblocks() { return gridDim.x * gridDim.y * gridDim.z; }
+__device__ inline int blin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
 __device__ inline double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
+  const int t = tlin();
+  red[t] = v;
   __syncthreads();
   for (int w = AB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if (t < w) red[t] += red[t + w];
     __syncthreads();
   }
   const double s = red[0];
@@ -97,87 +121,56 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
-// last-block-done: returns true in exactly one block (the last to arrive)
+// last-block-done: true in exactly one block (the last to arrive)
 __device__ inline bool last_block(AarState* S) {
   __shared__ int ticket;
   __threadfence();
-  if (threadIdx.x == 0) ticket = atomicAdd(&S->counter, 1);
+  if (tlin() == 0) ticket = atomicAdd(&S->counter, 1);
   __syncthreads();
-  const bool last = ticket == (int)gridDim.x - 1;
+  const bool last = ticket == nblocks() - 1;
   if (last) __threadfence();
   return last;
 }
 
-// ||f|| -> nb_inv
+// fixed-order sum of nd columns of part[nblk][nd] by the whole block into tot[nd]
+__device__ inline void reduce_partials(const double* part, int nblk, int nd, double* tot) {
+  __shared__ double tmp[AB];
+  const int lanes = AB / nd;
+  const int t = tlin();
+  if (t < nd * lanes) {
+    const int d = t / lanes, l = t % lanes;
+    double s = 0;
+    for (int b = l; b < nblk; b += lanes) s += part[(int64_t)b * nd + d];
+    tmp[t] = s;
+  }
+  __syncthreads();
+  if (t < nd) {
+    double s = 0;
+    for (int l = 0; l < lanes; ++l) s += tmp[t * lanes + l];
+    tot[t] = s;
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, int64_t N, double* part, AarState* S) {
   __shared__ double red[AB];
+  __shared__ double tot[1];
   double a = 0;
   for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) a += f[n] * f[n];
   const double s = block_sum(a, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
   if (last_block(S)) {
+    reduce_partials(part, nblocks(), 1, tot);
     if (threadIdx.x == 0) {
-      double t = 0;
-      for (int b = 0; b < (int)gridDim.x; ++b) t += part[b];
-      S->nb_inv = 1.0 / sqrt(t);
+      S->nb_inv = 1.0 / sqrt(tot[0]);
       S->counter = 0;
     }
   }
 }
 
-// res1 = f - A x ; relres ; res = res1 ./ diag
-__global__ void __launch_bounds__(AB) residual_kernel(PoissonOp P, const double* __restrict__ f,
-                                                      const double* __restrict__ x, double* __restrict__ res,
-                                                      double* part, AarState* S, int count, int max_iter, double tol,
-                                                      int xbuf) {
-  if (S->done) return;
-  __shared__ double red[AB];
-  const int64_t N = (int64_t)P.nr * P.nt * P.nz;
-  double a = 0;
-  for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) {
-    const int i = (int)(n % P.nr);
-    const int j = (int)((n / P.nr) % P.nt);
-    const int k = (int)(n / ((int64_t)P.nr * P.nt));
-    const double r1 = f[n] - lap_at(P, x, i, j, k);
-    a += r1 * r1;
-    res[n] = r1 / P.coef[i * RCOEF + 2 * FDN + 1];
-  }
-  const double s = block_sum(a, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (last_block(S)) {
-    if (threadIdx.x == 0) {
-      double t = 0;
-      for (int b = 0; b < (int)gridDim.x; ++b) t += part[b];
-      const double relres = sqrt(t) * S->nb_inv;
-      S->relres = relres;
-      S->iters = count;
-      S->final_buf = xbuf;
-      if (relres <= tol || count >= max_iter) S->done = (relres <= tol) ? 1 : 2;
-      S->counter = 0;
-    }
-  }
-}
-
-// history update + Richardson step (count % p != 0)
-__global__ void __launch_bounds__(AB) richardson_kernel(int64_t N, const double* __restrict__ xp,
-                                                        const double* __restrict__ res, const double* __restrict__ fold,
-                                                        double* xnew_xold, double* DX, double* DF, int hcol,
-                                                        double omega, const AarState* S) {
-  if (S->done) return;
-  for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) {
-    const double x = xp[n];
-    const double r = res[n];
-    if (hcol >= 0) {
-      DX[(int64_t)hcol * N + n] = x - xnew_xold[n];
-      DF[(int64_t)hcol * N + n] = r - fold[n];
-    }
-    xnew_xold[n] = x + omega * r;
-  }
-}
-
-// symmetric m x m pseudo-inverse times vector, MATLAB pinv tolerance
+// symmetric m x m pseudo-inverse times vector with MATLAB pinv's tolerance
 // max(size) * norm * eps, by cyclic Jacobi eigen-decomposition (one thread)
-__device__ void pinv_solve(int m, double* A /*m*m*/, const double* b, double* out) {
+__device__ void pinv_solve(int m, double* A, const double* b, double* out) {
   double V[MAXHIST * MAXHIST];
   for (int i = 0; i < m; ++i)
     for (int j = 0; j < m; ++j) V[i * m + j] = (i == j) ? 1.0 : 0.0;
@@ -193,8 +186,7 @@ __device__ void pinv_solve(int m, double* A /*m*m*/, const double* b, double* ou
       for (int q = p + 1; q < m; ++q) {
         const double apq = A[p * m + q];
         if (apq == 0.0) continue;
-        const double app = A[p * m + p], aqq = A[q * m + q];
-        const double theta = (aqq - app) / (2.0 * apq);
+        const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
         const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
         for (int k = 0; k < m; ++k) {
@@ -228,70 +220,120 @@ __device__ void pinv_solve(int m, double* A /*m*m*/, const double* b, double* ou
   }
 }
 
-// history update + DF^T DF, DF^T res partials; last block solves gamma
-__global__ void __launch_bounds__(AB) anderson_dots_kernel(int64_t N, const double* __restrict__ xp,
-                                                           const double* __restrict__ res,
-                                                           const double* __restrict__ fold,
-                                                           const double* __restrict__ xold, double* DX, double* DF,
-                                                           int hcol, int m, double* part, AarState* S) {
-  if (S->done) return;
+struct StepArgs {
+  const double* f;
+  const double* xp;     // x_prev (its residual is measured)
+  double* xo;           // Xold on entry; x_new on exit of a Richardson step
+  const double* T;      // theta derivative of x_prev
+  double* res;          // preconditioned residual of x_prev (out)
+  const double* fold;   // previous preconditioned residual
+  double* DX;
+  double* DF;
+  double* part;
+  AarState* S;
+  int hcol;             // history column written this iteration (-1: none)
+  int anderson;         // 1 on Anderson steps
+  int count, max_iter, xbuf;
+  double tol, omega;
+};
+
+// MH: compile-time history size for the Anderson dot products (0: none)
+template <int MH>
+__global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
+  if (A.S->done) return;
   __shared__ double red[AB];
-  const int nd = m * (m + 1) / 2 + m;
-  double acc[MAXHIST * (MAXHIST + 1) / 2 + MAXHIST];
-  for (int d = 0; d < nd; ++d) acc[d] = 0;
-  for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) {
-    const double r = res[n];
-    DX[(int64_t)hcol * N + n] = xp[n] - xold[n];
-    DF[(int64_t)hcol * N + n] = r - fold[n];
-    double df[MAXHIST];
-    for (int k = 0; k < m; ++k) df[k] = DF[(int64_t)k * N + n];
-    int d = 0;
-    for (int a = 0; a < m; ++a)
-      for (int b = a; b < m; ++b) acc[d++] += df[a] * df[b];
-    for (int a = 0; a < m; ++a) acc[d++] += df[a] * r;
-  }
-  for (int d = 0; d < nd; ++d) {
-    const double s = block_sum(acc[d], red);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nd + d] = s;
-  }
-  if (last_block(S)) {
-    __shared__ double tot[MAXHIST * (MAXHIST + 1) / 2 + MAXHIST];
-    for (int d = threadIdx.x; d < nd; d += AB) {
-      double t = 0;
-      for (int b = 0; b < (int)gridDim.x; ++b) t += part[(int64_t)b * nd + d];
-      tot[d] = t;
+  __shared__ double tot[MH * (MH + 1) / 2 + MH + 1];
+  constexpr int ND = MH * (MH + 1) / 2 + MH;
+  const int64_t N = (int64_t)P.nr * P.nt * P.nz;
+  double nrm = 0;
+  double acc[ND > 0 ? ND : 1];
+#pragma unroll
+  for (int d = 0; d < (ND > 0 ? ND : 1); ++d) acc[d] = 0;
+  const Pt q = point_of(P.nr, P.nt);
+  if (q.ok) {
+    const int n = q.n;
+    const double x = A.xp[n];
+    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
+    nrm = r1 * r1;
+    const double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
+    A.res[n] = r;
+    if (A.hcol >= 0) {
+      A.DX[(int64_t)A.hcol * N + n] = x - A.xo[n];
+      A.DF[(int64_t)A.hcol * N + n] = r - A.fold[n];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double A[MAXHIST * MAXHIST], bb[MAXHIST];
+    if (!A.anderson) {
+      A.xo[n] = __builtin_fma(A.omega, r, x);
+    } else if constexpr (MH > 0) {
+      double df[MH];
+#pragma unroll
+      for (int a = 0; a < MH; ++a) df[a] = A.DF[(int64_t)a * N + n];
       int d = 0;
-      for (int a = 0; a < m; ++a)
-        for (int b = a; b < m; ++b) {
-          A[a * m + b] = tot[d];
-          A[b * m + a] = tot[d];
-          ++d;
+#pragma unroll
+      for (int a = 0; a < MH; ++a)
+#pragma unroll
+        for (int b = a; b < MH; ++b) acc[d++] = df[a] * df[b];
+#pragma unroll
+      for (int a = 0; a < MH; ++a) acc[d++] = df[a] * r;
+    }
+  }
+  const int nd = 1 + (A.anderson ? ND : 0);
+  {
+    const double s = block_sum(nrm, red);
+    if (tlin() == 0) A.part[(int64_t)blin() * nd] = s;
+  }
+  if (A.anderson) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const double s = block_sum(acc[d], red);
+      if (tlin() == 0) A.part[(int64_t)blin() * nd + 1 + d] = s;
+    }
+  }
+  if (last_block(A.S)) {
+    reduce_partials(A.part, nblocks(), nd, tot);
+    if (tlin() == 0) {
+      AarState* S = A.S;
+      const double relres = sqrt(tot[0]) * S->nb_inv;
+      S->relres = relres;
+      S->iters = A.count;
+      S->final_buf = A.xbuf;
+      if (relres <= A.tol)
+        S->done = 1;
+      else if (A.count >= A.max_iter)
+        S->done = 2;
+      if (A.anderson && !S->done) {
+        if constexpr (MH > 0) {
+          double Am[MAXHIST * MAXHIST], bb[MAXHIST];
+          int d = 1;
+          for (int a = 0; a < MH; ++a)
+            for (int b = a; b < MH; ++b) {
+              Am[a * MH + b] = tot[d];
+              Am[b * MH + a] = tot[d];
+              ++d;
+            }
+          for (int a = 0; a < MH; ++a) bb[a] = tot[d++];
+          pinv_solve(MH, Am, bb, S->gamma);
         }
-      for (int a = 0; a < m; ++a) bb[a] = tot[d++];
-      pinv_solve(m, A, bb, S->gamma);
+      }
       S->counter = 0;
     }
   }
 }
 
-// x_new = x_prev + beta*res - (DX + beta*DF) gamma
-__global__ void __launch_bounds__(AB) anderson_update_kernel(int64_t N, const double* __restrict__ xp,
+// x_new = x_prev + beta res - (DX + beta DF) gamma   (src/aar.m:59-60)
+template <int MH>
+__global__ void __launch_bounds__(AB) anderson_update_kernel(int nr, int nt, int64_t N, const double* __restrict__ xp,
                                                              const double* __restrict__ res, double* __restrict__ xnew,
                                                              const double* __restrict__ DX,
-                                                             const double* __restrict__ DF, int m, double beta,
+                                                             const double* __restrict__ DF, double beta,
                                                              const AarState* S) {
   if (S->done) return;
-  double g[MAXHIST];
-  for (int k = 0; k < m; ++k) g[k] = S->gamma[k];
-  for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) {
-    double corr = 0;
-    for (int k = 0; k < m; ++k) corr += (DX[(int64_t)k * N + n] + beta * DF[(int64_t)k * N + n]) * g[k];
-    xnew[n] = xp[n] + beta * res[n] - corr;
-  }
+  const Pt q = point_of(nr, nt);
+  if (!q.ok) return;
+  const int n = q.n;
+  double corr = 0;
+#pragma unroll
+  for (int k = 0; k < MH; ++k) corr += (DX[(int64_t)k * N + n] + beta * DF[(int64_t)k * N + n]) * S->gamma[k];
+  xnew[n] = xp[n] + beta * res[n] - corr;
 }
 
 __global__ void fill_kernel(double* x, int64_t N, double v) {
@@ -312,17 +354,27 @@ int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f
   return CYX_OK;
 }
 
+template <int MH>
+static void launch_anderson(dim3 grid, hipStream_t st, const PoissonOp& P, const StepArgs& A, int64_t N, double beta,
+                            double* xnew) {
+  hipLaunchKernelGGL(aar_step_kernel<MH>, grid, dim3(TX, TY), 0, st, P, A);
+  hipLaunchKernelGGL(anderson_update_kernel<MH>, grid, dim3(TX, TY), 0, st, P.nr, P.nt, N, A.xp, (const double*)A.res, xnew,
+                     (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
+}
+
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
                 double omega, double beta, int m, int p, cyx_aar_info* info) {
-  CYX_CHECK(m >= 1 && m <= MAXHIST, "AAR history m must be in [1, %d]", MAXHIST);
+  CYX_CHECK(m >= 1 && m <= 8, "AAR history m must be in [1, 8]");
   CYX_CHECK(p >= 1 && max_iter >= 1 && tol > 0, "invalid AAR parameters");
   const int64_t N = ps->N();
   hipStream_t st = ps->ctx->stream;
-  const int nblk = (int)std::min<int64_t>(ceil_div(N, AB), 2048);
-  const int nd = m * (m + 1) / 2 + m;
-  // scratch: xalt | res0 | res1 | DX | DF | partials | state
+  const dim3 pgrid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
+  const int nblk = (int)(pgrid.x * pgrid.y * pgrid.z);
+  const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
+  const int ndmax = 1 + m * (m + 1) / 2 + m;
+  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state
   const size_t vec = (size_t)N * sizeof(double);
-  const size_t need = vec * (3 + 2 * (size_t)m) + (size_t)nblk * nd * sizeof(double) + sizeof(AarState) + 256;
+  const size_t need = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
   if (ps->scratch_bytes < need) {
     if (ps->scratch) (void)hipFree(ps->scratch);
     ps->scratch = nullptr;
@@ -335,67 +387,82 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
   }
   double* xalt = ps->scratch;
   double* R[2] = {xalt + N, xalt + 2 * N};
-  double* DX = xalt + 3 * N;
+  double* T = xalt + 3 * N;
+  double* DX = xalt + 4 * N;
   double* DF = DX + (int64_t)m * N;
   double* part = DF + (int64_t)m * N;
-  AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * nd) + 255) & ~(uintptr_t)255);
-  // X[1] = caller's x (guess), X[0] = xalt
-  double* X[2] = {xalt, x};
+  AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * ndmax) + 255) & ~(uintptr_t)255);
+  double* X[2] = {xalt, x};  // X[1] = caller's x (guess), X[0] = xalt
   if (!use_guess) hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, x, N, 1.0);
   CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
+  CYX_HIP(hipMemsetAsync(T, 0, vec, st));
   CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
-  hipLaunchKernelGGL(norm_kernel, dim3(nblk), dim3(AB), 0, st, f, N, part, S);
+  hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part, S);
   PoissonOp P{};
   P.nr = ps->g.nr;
   P.nt = ps->g.nt;
   P.nz = ps->g.nz;
   P.coef = ps->coef;
+  P.helical = (ps->g.cell_typ == 4 || ps->g.cell_typ == 5) ? 1 : 0;
   {
-    double w2[FDN + 1], w1[FDN + 1];
-    // same closed form as the host set-up (src/initialization.m:403-419)
-    double fact[2 * FDN + 2];
+    double fact[2 * FDN + 2], w2[FDN + 1], w1[FDN + 1];
     fact[0] = 1;
     for (int k = 1; k < 2 * FDN + 2; ++k) fact[k] = fact[k - 1] * k;
-    w2[0] = 0;
-    w1[0] = 0;
-    for (int k = 1; k <= FDN; ++k) {
+    w2[0] = w1[0] = 0;
+    for (int k = 1; k <= FDN; ++k) {  // src/initialization.m:403-419
       const double c = fact[FDN] * fact[FDN] / (fact[FDN - k] * fact[FDN + k]);
       w2[k] = 2 * ((k % 2) ? 1.0 : -1.0) * c / (k * k);
       w1[k] = ((k % 2) ? 1.0 : -1.0) * c / k;
       w2[0] -= 2.0 / (k * k);
     }
-    const double tau = ps->g.twist;
     for (int k = 0; k <= FDN; ++k) {
-      P.wt2[k] = w2[k];
+      P.w2[k] = w2[k];
       P.wz2[k] = w2[k] / (ps->g.dz * ps->g.dz);
-      for (int q = 0; q <= FDN; ++q) P.cx[k][q] = (-2.0 * tau * w1[k] / ps->g.dt) * (w1[q] / ps->g.dz);
+      P.wt1[k] = -2.0 * ps->g.twist * w1[k] / ps->g.dt;
+      P.wz1[k] = w1[k] / ps->g.dz;
     }
-    P.helical = (ps->g.cell_typ == 4 || ps->g.cell_typ == 5) ? 1 : 0;
   }
   AarState hs{};
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
   const int chunk = 16;
+  const dim3 grid = pgrid;
   while (count <= max_iter) {
     const int cend = std::min(max_iter, count + chunk - 1);
     for (; count <= cend; ++count) {
       const int xb = count & 1;
-      double* xp = X[xb];
-      double* xo = X[xb ^ 1];  // Xold on entry, x_new on exit
-      double* res = R[xb];
-      double* fold = R[xb ^ 1];
-      hipLaunchKernelGGL(residual_kernel, dim3(nblk), dim3(AB), 0, st, P, f, (const double*)xp, res, part, S, count,
-                         max_iter, tol, xb);
-      const int hcol = count > 1 ? (count - 2) % m : -1;
-      if (count % p != 0) {
-        hipLaunchKernelGGL(richardson_kernel, dim3(nblk), dim3(AB), 0, st, N, (const double*)xp, (const double*)res,
-                           (const double*)fold, xo, DX, DF, hcol, omega, (const AarState*)S);
+      StepArgs A{};
+      A.f = f;
+      A.xp = X[xb];
+      A.xo = X[xb ^ 1];
+      A.T = T;
+      A.res = R[xb];
+      A.fold = R[xb ^ 1];
+      A.DX = DX;
+      A.DF = DF;
+      A.part = part;
+      A.S = S;
+      A.hcol = count > 1 ? (count - 2) % m : -1;
+      A.anderson = (count % p == 0) ? 1 : 0;
+      A.count = count;
+      A.max_iter = max_iter;
+      A.xbuf = xb;
+      A.tol = tol;
+      A.omega = omega;
+      if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, A.xp, T, (const AarState*)S);
+      if (!A.anderson) {
+        hipLaunchKernelGGL(aar_step_kernel<0>, grid, dim3(TX, TY), 0, st, P, A);
       } else {
-        hipLaunchKernelGGL(anderson_dots_kernel, dim3(nblk), dim3(AB), 0, st, N, (const double*)xp,
-                           (const double*)res, (const double*)fold, (const double*)xo, DX, DF, hcol, m, part, S);
-        hipLaunchKernelGGL(anderson_update_kernel, dim3(nblk), dim3(AB), 0, st, N, (const double*)xp,
-                           (const double*)res, xo, (const double*)DX, (const double*)DF, m, beta,
-                           (const AarState*)S);
+        switch (m) {
+          case 1: launch_anderson<1>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 2: launch_anderson<2>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 3: launch_anderson<3>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 4: launch_anderson<4>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 5: launch_anderson<5>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 6: launch_anderson<6>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 7: launch_anderson<7>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          default: launch_anderson<8>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+        }
       }
     }
     CYX_HIP(hipGetLastError());

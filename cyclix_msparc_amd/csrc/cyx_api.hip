@@ -412,26 +412,56 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   if ((rc = gram_w(h, dtype, psi, other, ns, N, Hs)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, Hs, ns)) != CYX_OK) return rc;
-  rocblas_status bs;
-  PhaseTimer tm(h->ctx, PH_DENSE_EIG);
-  if (cx)
-    bs = rocsolver_zhegvd((rocblas_handle)h->ctx->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, ns,
-                          (rocblas_double_complex*)Hs, ns, (rocblas_double_complex*)M, ns, D, E, info);
-  else
-    bs = rocsolver_dsygvd((rocblas_handle)h->ctx->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, ns, (double*)Hs,
-                          ns, (double*)M, ns, D, E, info);
-  if (bs != rocblas_status_success) {
-    set_error("rocsolver hegvd/sygvd failed with status %d", (int)bs);
-    return CYX_ERR_SOLVER;
+  {
+    // Hermitian-definite pencil (Hs, M) in FP64: M = U^H U, Hs' = U^-H Hs U^-1,
+    // Hs' = Q' D Q'^H, Q = U^-1 Q' (Q^H M Q = I).
+    PhaseTimer tm(h->ctx, PH_DENSE_EIG);
+    rocblas_handle bh = (rocblas_handle)h->ctx->blas;
+    rocblas_status bs[5];
+    if (cx) {
+      const rocblas_double_complex one = {1.0, 0.0};
+      auto* m = (rocblas_double_complex*)M;
+      auto* a = (rocblas_double_complex*)Hs;
+      bs[0] = rocsolver_zpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
+      bs[1] = rocblas_ztrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      bs[2] = rocblas_ztrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      bs[3] = rocsolver_zheevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      bs[4] = rocblas_ztrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+    } else {
+      const double one = 1.0;
+      auto* m = (double*)M;
+      auto* a = (double*)Hs;
+      bs[0] = rocsolver_dpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
+      bs[1] = rocblas_dtrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      bs[2] = rocblas_dtrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      bs[3] = rocsolver_dsyevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      bs[4] = rocblas_dtrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                            rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+    }
+    for (int k = 0; k < 5; ++k)
+      if (bs[k] != rocblas_status_success) {
+        set_error("dense Rayleigh-Ritz step %d failed with rocBLAS/rocSOLVER status %d", k, (int)bs[k]);
+        return CYX_ERR_SOLVER;
+      }
   }
   if ((rc = rotate(h, dtype, psi, Hs, other, ns, N)) != CYX_OK) return rc;
   if ((rc = wnormalize(h, dtype, other, ns, N, scal)) != CYX_OK) return rc;
-  rocblas_int hinfo = 0;
+  rocblas_int hinfo[2] = {0, 0};
   CYX_HIP(hipMemcpyAsync(eigval, D, ns * sizeof(double), hipMemcpyDeviceToHost, st));
-  CYX_HIP(hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CYX_HIP(hipMemcpyAsync(hinfo, info, 2 * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
   CYX_HIP(hipStreamSynchronize(st));
-  if (hinfo != 0) {
-    set_error("generalized eigensolver info = %d (overlap not positive definite or no convergence)", (int)hinfo);
+  if (hinfo[0] != 0) {
+    set_error("overlap matrix not positive definite (potrf info = %d): the filtered block is rank deficient",
+              (int)hinfo[0]);
+    return CYX_ERR_SOLVER;
+  }
+  if (hinfo[1] != 0) {
+    set_error("projected eigensolver did not converge (heevd info = %d)", (int)hinfo[1]);
     return CYX_ERR_SOLVER;
   }
   *result = other;
@@ -677,6 +707,15 @@ int cyx_ham_destroy(cyx_ham_t h) {
 }
 
 const double* cyx_ham_weights(cyx_ham_t h) { return h ? h->W : nullptr; }
+
+int cyx_ham_set_tuning(cyx_ham_t h, const cyx_tuning* t) {
+  CYX_CHECK(h && t, "null argument");
+  CYX_CHECK(t->tr >= 0 && t->zb >= 0 && t->flags >= 0 && t->flags < 4, "bad tuning values");
+  h->tune_tr = t->tr;
+  h->tune_zb = t->zb;
+  h->tune_flags = t->flags;
+  return CYX_OK;
+}
 
 int cyx_h_apply(cyx_ham_t h, int dtype, const void* X, void* HX, int ncol, int64_t ld) {
   CYX_CHECK(h && X && HX, "null argument");

@@ -103,6 +103,8 @@ struct cyx_ham_s {
   cyx::KbTables<cyx::cplx<double>> kb_z;
   cyx::KbTables<cyx::cplx<float>> kb_c;
   int nproj = 0;
+  // stencil launch shape (0: heuristic), see cyx_tuning
+  int tune_tr = 0, tune_zb = 0, tune_flags = 0;
   // lazily grown scratch
   cyx::DeviceBuf kbC;      // C coefficients (ncol x nproj x 16 B)
   cyx::DeviceBuf small;    // Gram partial sums

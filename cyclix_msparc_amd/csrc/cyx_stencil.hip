@@ -37,6 +37,7 @@ struct StepParams {
   int tr, zb;
   int ntiles_r, nblocks_z;
   int xcd_remap;
+  int nontemporal;
   R alpha, beta, shift;
   const R* rcoef;
   const R* veff;
@@ -50,6 +51,24 @@ struct StepParams {
   const T* kbC;
   int nproj;
 };
+
+// non-temporal (streaming) element access: the X_prev stream and the result
+// are touched once, so they should not displace halo planes and Veff in L2
+__device__ __forceinline__ double ntload(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ntload(const float* p) { return __builtin_nontemporal_load(p); }
+template <class R>
+__device__ __forceinline__ cplx<R> ntload(const cplx<R>* p) {
+  const R* q = reinterpret_cast<const R*>(p);
+  return cplx<R>{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+__device__ __forceinline__ void ntstore(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void ntstore(float* p, float v) { __builtin_nontemporal_store(v, p); }
+template <class R>
+__device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
+  R* q = reinterpret_cast<R*>(p);
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+}
 
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(const StepParams<T> P) {
@@ -163,7 +182,8 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
         const int64_t nout = (int64_t)ko * nrt + (int64_t)j * P.nr + r;
         T pv = zero<T>();
         if constexpr (HAS_PREV) {
-          if (emit && rvalid) pv = P.prev[colbase + nout];
+          if (emit && rvalid)
+            pv = P.nontemporal ? ntload(P.prev + colbase + nout) : P.prev[colbase + nout];
         }
         // ---- compute plane kk
         const int c = (j + FDN) * TRH + (i + FDN);
@@ -221,7 +241,10 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
             }
             T y = rmul(P.alpha, h);
             if constexpr (HAS_PREV) y = rfma(-P.beta, pv, y);
-            P.out[colbase + nout] = y;
+            if (P.nontemporal)
+              ntstore(P.out + colbase + nout, y);
+            else
+              P.out[colbase + nout] = y;
           }
           qo = zero<T>();
         }
@@ -327,10 +350,13 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.nt = g.nt;
   P.nz = g.nz;
   P.tr = choose_tr(g.nr, g.nt, stencil_max_threads<T>());
+  if (h->tune_tr > 0) P.tr = std::max(2 * FDN, std::min(h->tune_tr, stencil_max_threads<T>() / g.nt));
   P.ntiles_r = ceil_div(g.nr, P.tr);
   int zb = g.nz;
   while ((long)P.ntiles_r * ceil_div(g.nz, zb) * ncol < 512 && zb > 4) zb = (zb + 1) / 2;
+  if (h->tune_zb > 0) zb = std::min(h->tune_zb, g.nz);
   P.zb = zb;
+  P.nontemporal = (h->tune_flags & 1) ? 1 : 0;
   P.nblocks_z = ceil_div(g.nz, zb);
   P.alpha = (R)alpha;
   P.beta = (R)beta;
@@ -358,7 +384,7 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.nproj = K.nproj;
   const long nblk = (long)P.ntiles_r * P.nblocks_z * ncol;
   CYX_CHECK(nblk < (1L << 31), "grid too large");
-  P.xcd_remap = (nblk % 8 == 0) ? 1 : 0;
+  P.xcd_remap = (nblk % 8 == 0 && !(h->tune_flags & 2)) ? 1 : 0;
   const int nthr = P.tr * g.nt;
   const size_t shm = 2 * (size_t)(P.tr + 2 * FDN) * (g.nt + 2 * FDN) * sizeof(T);
   const bool hel = g.cell_typ == 4 || g.cell_typ == 5;

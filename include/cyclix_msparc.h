@@ -111,6 +111,16 @@ int cyx_ham_destroy(cyx_ham_t ham);
 /* Device pointer of the FP64 weights W held by the handle (length N). */
 const double* cyx_ham_weights(cyx_ham_t ham);
 
+/* Launch shape of the fused stencil kernel (the autotuner's knobs; no
+ * counterpart in the reference).  tr: radial tile (0 = heuristic; clamped to
+ * the kernel's thread limit), zb: z planes per workgroup (0 = heuristic),
+ * flags: bit 0 non-temporal loads of X_prev and stores of the result, bit 1
+ * disable the XCD-aware block remap.  Results do not depend on these. */
+typedef struct cyx_tuning {
+  int32_t tr, zb, flags, reserved;
+} cyx_tuning;
+int cyx_ham_set_tuning(cyx_ham_t ham, const cyx_tuning* t);
+
 /* HX = H X.  Replaces h_nonlocal_vector_mult (src/h_nonlocal_vector_mult.m:1-43).
  * X, HX device, column-major N x ncol, leading dimension ld. */
 int cyx_h_apply(cyx_ham_t ham, int dtype, const void* X, void* HX, int ncol, int64_t ld);
