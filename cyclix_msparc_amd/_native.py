@@ -24,7 +24,7 @@ EXPORTS = (
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_eigsolver_workspace_bytes", "cyx_eigsolver_pass",
     "cyx_rayleigh_ritz", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
-    "cyx_poisson_solve",
+    "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
 )
 
 
@@ -98,6 +98,10 @@ def load():
         "cyx_poisson_destroy": (i32, [vp]),
         "cyx_aar": (i32, [vp, vp, vp, i32, dbl, i32, dbl, dbl, i32, i32, C.POINTER(AarInfo)]),
         "cyx_poisson_solve": (i32, [vp, vp, vp, vp, i32, dbl, i32, C.POINTER(AarInfo)]),
+        "cyx_scf_terms": (i32, [vp, vp, vp, vp, pd]),
+        "cyx_mixer_create": (i32, [vp, i64, i32, dbl, dbl, i32, vp, C.POINTER(vp)]),
+        "cyx_mixer_destroy": (i32, [vp]),
+        "cyx_mixer_step": (i32, [vp, vp, vp, vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

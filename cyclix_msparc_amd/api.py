@@ -231,6 +231,14 @@ class Hamiltonian:
                                             nat.dptr(out)))
         return tuple(out)
 
+    def scf_terms(self, vout, vin, rho):
+        """(Escc, ||vout - vin||^2, ||vout||^2) for potential mixing
+        (``src/scf.m:160-162,182-184``)."""
+        out = np.zeros(3)
+        nat.check(self.lib.cyx_scf_terms(self.h, C.c_void_p(vout.data_ptr()), C.c_void_p(vin.data_ptr()),
+                                         C.c_void_p(rho.data_ptr()), nat.dptr(out)))
+        return tuple(out)
+
     def lanczos_bounds(self, v0, maxit=300, tol=1e-3):
         v0 = np.ascontiguousarray(np.asarray(v0, dtype=float))
         lo, hi = C.c_double(), C.c_double()
@@ -248,6 +256,48 @@ def lda_pw(dev: Device, rho, phi=None, rhotol=1e-14):
                                  C.c_void_p(vxc.data_ptr()), C.c_void_p(veff.data_ptr()) if veff is not None else None,
                                  float(rhotol)))
     return vxc, veff
+
+
+class Mixer:
+    """Periodic-Pulay mixing state (``src/mixing.m:20-116``) for FP64 device
+    vectors of length N; ``x0`` is the initial input iterate
+    (``src/scf.m:42-56``)."""
+
+    def __init__(self, dev: Device, x0, history=7, beta=0.3, omega=None, pulay_frequency=1):
+        self.dev = dev
+        self.lib = dev.lib
+        x0 = x0.contiguous()
+        if x0.dtype != torch.float64 or not x0.is_cuda:
+            raise ValueError("x0 must be a float64 device tensor")
+        self.N = x0.numel()
+        h = C.c_void_p()
+        omega = beta if omega is None or omega < 0 else omega   # src/initialization.m:573-575
+        nat.check(self.lib.cyx_mixer_create(dev.h, self.N, int(history), float(beta), float(omega),
+                                            int(pulay_frequency), C.c_void_p(x0.data_ptr()), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cyx_mixer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, g_k, x_k, it, out=None):
+        """x_{k+1} = mixing(S, g_k, x_k, iter) (``src/mixing.m:20``)."""
+        for t in (g_k, x_k):
+            if t.dtype != torch.float64 or t.numel() != self.N or not t.is_contiguous():
+                raise ValueError("mixing operands must be contiguous float64 of length N")
+        out = torch.empty_like(x_k) if out is None else out
+        if out.data_ptr() == x_k.data_ptr():
+            raise ValueError("x_out may not alias x_k")
+        nat.check(self.lib.cyx_mixer_step(self.h, C.c_void_p(g_k.data_ptr()), C.c_void_p(x_k.data_ptr()),
+                                          C.c_void_p(out.data_ptr()), int(it)))
+        return out
 
 
 class Poisson:

@@ -3,18 +3,20 @@
 // extras/AAR_old.m:45-46,73-75).  Operator: S.Lap_std at k = 0
 // (src/initialization.m:734-741), applied matrix-free.
 //
-// One AAR iteration is two launches (three on Anderson steps):
+// One AAR iteration is three launches (four on Anderson steps):
 //   1. theta_kernel: T = (-2 tau D_theta) x  (helical cross term, separable)
-//   2. step kernel:  res1 = f - Lap x, relres partials, res = res1 ./ diag,
-//      history columns, and on Richardson steps x_new = x + omega res; the
-//      last block to finish sums the partials in a fixed order, decides
-//      convergence and, on Anderson steps, forms DF'DF, DF'res and solves
-//      gamma = pinv(DF'DF) DF'res;
-//   3. (Anderson) x_new = x + beta res - (DX + beta DF) gamma.
+//   2. step kernel:  res1 = f - Lap x, res = res1 ./ diag, history columns,
+//      on Richardson steps x_new = x + omega res (written speculatively into
+//      the other x buffer), per-block partials of ||res1||^2 and, on
+//      Anderson steps, of DF'DF and DF'res;
+//   3. finalize (one block): fixed-order sums, convergence flag, and on
+//      Anderson steps gamma = pinv(DF'DF) DF'res;
+//   4. (Anderson) x_new = x + beta res - (DX + beta DF) gamma.
 // Kernels of an iteration that starts after convergence return at once, so
 // the host enqueues iterations in chunks and polls a device flag between
 // chunks.  Every reduction has a fixed order: a solve is bitwise reproducible.
 #include "cyx_internal.hpp"
+#include "cyx_reduce.hpp"
 
 namespace cyx {
 
@@ -28,8 +30,6 @@ struct AarState {
   double gamma[16];
 };
 
-constexpr int AB = 256;
-constexpr int MAXHIST = 15;
 
 struct PoissonOp {
   int nr, nt, nz;
@@ -103,121 +103,18 @@ __device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x
   return acc;
 }
 
-// sum of v over the block (AB threads), result valid in every thread
-__device__ inline int tlin() { return threadIdx.x + blockDim.x * threadIdx.y; }
-__device__ inline int nblocks() { return gridDim.x * gridDim.y * gridDim.z; }
-__device__ inline int blin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-
-__device__ inline double block_sum(double v, double* red) {
-  const int t = tlin();
-  red[t] = v;
-  __syncthreads();
-  for (int w = AB / 2; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
-  }
-  const double s = red[0];
-  __syncthreads();
-  return s;
-}
-
-// last-block-done: true in exactly one block (the last to arrive)
-__device__ inline bool last_block(AarState* S) {
-  __shared__ int ticket;
-  __threadfence();
-  if (tlin() == 0) ticket = atomicAdd(&S->counter, 1);
-  __syncthreads();
-  const bool last = ticket == nblocks() - 1;
-  if (last) __threadfence();
-  return last;
-}
-
-// fixed-order sum of nd columns of part[nblk][nd] by the whole block into tot[nd]
-__device__ inline void reduce_partials(const double* part, int nblk, int nd, double* tot) {
-  __shared__ double tmp[AB];
-  const int lanes = AB / nd;
-  const int t = tlin();
-  if (t < nd * lanes) {
-    const int d = t / lanes, l = t % lanes;
-    double s = 0;
-    for (int b = l; b < nblk; b += lanes) s += part[(int64_t)b * nd + d];
-    tmp[t] = s;
-  }
-  __syncthreads();
-  if (t < nd) {
-    double s = 0;
-    for (int l = 0; l < lanes; ++l) s += tmp[t * lanes + l];
-    tot[t] = s;
-  }
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, int64_t N, double* part, AarState* S) {
+__global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, int64_t N, double* part) {
   __shared__ double red[AB];
-  __shared__ double tot[1];
   double a = 0;
   for (int64_t n = (int64_t)blockIdx.x * AB + threadIdx.x; n < N; n += (int64_t)gridDim.x * AB) a += f[n] * f[n];
   const double s = block_sum(a, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (last_block(S)) {
-    reduce_partials(part, nblocks(), 1, tot);
-    if (threadIdx.x == 0) {
-      S->nb_inv = 1.0 / sqrt(tot[0]);
-      S->counter = 0;
-    }
-  }
 }
 
-// symmetric m x m pseudo-inverse times vector with MATLAB pinv's tolerance
-// max(size) * norm * eps, by cyclic Jacobi eigen-decomposition (one thread)
-__device__ void pinv_solve(int m, double* A, const double* b, double* out) {
-  double V[MAXHIST * MAXHIST];
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) V[i * m + j] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0, tot = 0;
-    for (int i = 0; i < m; ++i)
-      for (int j = 0; j < m; ++j) {
-        tot += A[i * m + j] * A[i * m + j];
-        if (i != j) off += A[i * m + j] * A[i * m + j];
-      }
-    if (off <= 1e-32 * tot || off == 0.0) break;
-    for (int p = 0; p < m - 1; ++p)
-      for (int q = p + 1; q < m; ++q) {
-        const double apq = A[p * m + q];
-        if (apq == 0.0) continue;
-        const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < m; ++k) {
-          const double akp = A[k * m + p], akq = A[k * m + q];
-          A[k * m + p] = c * akp - s * akq;
-          A[k * m + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < m; ++k) {
-          const double apk = A[p * m + k], aqk = A[q * m + k];
-          A[p * m + k] = c * apk - s * aqk;
-          A[q * m + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < m; ++k) {
-          const double vkp = V[k * m + p], vkq = V[k * m + q];
-          V[k * m + p] = c * vkp - s * vkq;
-          V[k * m + q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  double smax = 0;
-  for (int i = 0; i < m; ++i) smax = fmax(smax, fabs(A[i * m + i]));
-  const double tol = m * smax * 2.220446049250313e-16;
-  for (int i = 0; i < m; ++i) out[i] = 0;
-  for (int e = 0; e < m; ++e) {
-    const double lam = A[e * m + e];
-    if (fabs(lam) <= tol) continue;
-    double vb = 0;
-    for (int k = 0; k < m; ++k) vb += V[k * m + e] * b[k];
-    vb /= lam;
-    for (int k = 0; k < m; ++k) out[k] += V[k * m + e] * vb;
-  }
+__global__ void __launch_bounds__(AB) norm_finalize_kernel(const double* part, int nblk, AarState* S) {
+  __shared__ double tot[1];
+  reduce_partials(part, nblk, 1, tot);
+  if (threadIdx.x == 0) S->nb_inv = 1.0 / sqrt(tot[0]);
 }
 
 struct StepArgs {
@@ -242,7 +139,6 @@ template <int MH>
 __global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
   if (A.S->done) return;
   __shared__ double red[AB];
-  __shared__ double tot[MH * (MH + 1) / 2 + MH + 1];
   constexpr int ND = MH * (MH + 1) / 2 + MH;
   const int64_t N = (int64_t)P.nr * P.nt * P.nz;
   double nrm = 0;
@@ -288,33 +184,45 @@ __global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
       if (tlin() == 0) A.part[(int64_t)blin() * nd + 1 + d] = s;
     }
   }
-  if (last_block(A.S)) {
-    reduce_partials(A.part, nblocks(), nd, tot);
-    if (tlin() == 0) {
-      AarState* S = A.S;
-      const double relres = sqrt(tot[0]) * S->nb_inv;
-      S->relres = relres;
-      S->iters = A.count;
-      S->final_buf = A.xbuf;
-      if (relres <= A.tol)
-        S->done = 1;
-      else if (A.count >= A.max_iter)
-        S->done = 2;
-      if (A.anderson && !S->done) {
-        if constexpr (MH > 0) {
-          double Am[MAXHIST * MAXHIST], bb[MAXHIST];
-          int d = 1;
-          for (int a = 0; a < MH; ++a)
-            for (int b = a; b < MH; ++b) {
-              Am[a * MH + b] = tot[d];
-              Am[b * MH + a] = tot[d];
-              ++d;
-            }
-          for (int a = 0; a < MH; ++a) bb[a] = tot[d++];
-          pinv_solve(MH, Am, bb, S->gamma);
-        }
+}
+
+// one block: fixed-order sum of the step's partials, convergence decision
+// (src/aar.m:43-45) and, on Anderson steps, gamma = pinv(DF'DF) DF'res
+template <int MH>
+__global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, int nblk, int anderson, int count,
+                                                          int max_iter, int xbuf, double tol, AarState* S) {
+  if (S->done) return;
+  constexpr int ND = MH * (MH + 1) / 2 + MH;
+  __shared__ double tot[ND + 1];
+  const int nd = 1 + (anderson ? ND : 0);
+  reduce_partials(part, nblk, nd, tot);
+  if (threadIdx.x == 0) {
+    const double relres = sqrt(tot[0]) * S->nb_inv;
+    S->relres = relres;
+    S->iters = count;
+    S->final_buf = xbuf;
+    if (relres <= tol)
+      S->done = 1;
+    else if (count >= max_iter)
+      S->done = 2;
+    if constexpr (MH > 0) {
+      if (anderson && !S->done) {
+        double Am[MH * MH], bb[MH], g[MH];
+        int d = 1;
+#pragma unroll
+        for (int a = 0; a < MH; ++a)
+#pragma unroll
+          for (int b = a; b < MH; ++b) {
+            Am[a * MH + b] = tot[d];
+            Am[b * MH + a] = tot[d];
+            ++d;
+          }
+#pragma unroll
+        for (int a = 0; a < MH; ++a) bb[a] = tot[d++];
+        pinv_solve<MH>(Am, bb, g);
+#pragma unroll
+        for (int a = 0; a < MH; ++a) S->gamma[a] = g[a];
       }
-      S->counter = 0;
     }
   }
 }
@@ -355,11 +263,13 @@ int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f
 }
 
 template <int MH>
-static void launch_anderson(dim3 grid, hipStream_t st, const PoissonOp& P, const StepArgs& A, int64_t N, double beta,
-                            double* xnew) {
+static void launch_anderson(dim3 grid, int nblk, hipStream_t st, const PoissonOp& P, const StepArgs& A, int64_t N,
+                            double beta, double* xnew) {
   hipLaunchKernelGGL(aar_step_kernel<MH>, grid, dim3(TX, TY), 0, st, P, A);
-  hipLaunchKernelGGL(anderson_update_kernel<MH>, grid, dim3(TX, TY), 0, st, P.nr, P.nt, N, A.xp, (const double*)A.res, xnew,
-                     (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
+  hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nblk, 1, A.count,
+                     A.max_iter, A.xbuf, A.tol, A.S);
+  hipLaunchKernelGGL(anderson_update_kernel<MH>, grid, dim3(TX, TY), 0, st, P.nr, P.nt, N, A.xp, (const double*)A.res,
+                     xnew, (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
 }
 
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
@@ -397,7 +307,8 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
   CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
   CYX_HIP(hipMemsetAsync(T, 0, vec, st));
   CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
-  hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part, S);
+  hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part);
+  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk_norm, S);
   PoissonOp P{};
   P.nr = ps->g.nr;
   P.nt = ps->g.nt;
@@ -452,16 +363,18 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
       if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, A.xp, T, (const AarState*)S);
       if (!A.anderson) {
         hipLaunchKernelGGL(aar_step_kernel<0>, grid, dim3(TX, TY), 0, st, P, A);
+        hipLaunchKernelGGL(aar_finalize_kernel<0>, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, 0, count,
+                           max_iter, xb, tol, S);
       } else {
         switch (m) {
-          case 1: launch_anderson<1>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 2: launch_anderson<2>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 3: launch_anderson<3>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 4: launch_anderson<4>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 5: launch_anderson<5>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 6: launch_anderson<6>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 7: launch_anderson<7>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
-          default: launch_anderson<8>(grid, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 1: launch_anderson<1>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 2: launch_anderson<2>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 3: launch_anderson<3>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 4: launch_anderson<4>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 5: launch_anderson<5>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 6: launch_anderson<6>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 7: launch_anderson<7>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          default: launch_anderson<8>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
         }
       }
     }

@@ -129,9 +129,10 @@ static int upload_coef(OpCoef<R>& C, const cyx_grid_desc& g, const std::vector<d
 
 template <class T>
 struct HostKb {
-  std::vector<int> g_ptr, g_node, s_ptr, s_proj;
-  std::vector<T> g_coef, s_coef;
-  std::vector<typename traits<T>::real> gamma;
+  using R = typename traits<T>::real;
+  std::vector<int> g_ptr, g_node, s_ptr, s_il, il_proj;
+  std::vector<T> g_coef, il_phase;
+  std::vector<R> gamma, s_chi;
 };
 
 template <class T>
@@ -151,8 +152,8 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
   K.gamma.resize(nproj);
   for (int p = 0; p < nproj; ++p) K.gamma[p] = (R)d->gamma[p];
   struct Ent {
-    int node, proj;
-    double gr, gi, sr, si;
+    int node, il;
+    double chi, gr, gi;
   };
   std::vector<std::vector<Ent>> byproj(nproj);
   std::vector<int> cnt(N + 1, 0);
@@ -165,11 +166,14 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
     const double c = std::cos(ph), s = std::sin(ph);
     for (int l = 0; l < d->angnum[a]; ++l) {
       const int p = pbase[a] + l;
+      const int il = (int)K.il_proj.size();
+      K.il_proj.push_back(p);
+      K.il_phase.push_back(mk<T>(c, -s));
       for (int e = 0; e < nn; ++e) {
         const int node = d->rc_pos[off + e];
         const double chi = d->chi[choff + (int64_t)l * nn + e];
         const double gw = chi * W[node];
-        byproj[p].push_back(Ent{node, p, gw * c, gw * s, chi * c, -chi * s});
+        byproj[p].push_back(Ent{node, il, chi, gw * c, gw * s});
         cnt[node + 1]++;
       }
     }
@@ -186,14 +190,14 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
   }
   K.s_ptr.assign(N + 1, 0);
   for (int64_t n = 0; n < N; ++n) K.s_ptr[n + 1] = K.s_ptr[n] + cnt[n + 1];
-  K.s_proj.assign(K.s_ptr[N], 0);
-  K.s_coef.assign(K.s_ptr[N], T{});
+  K.s_chi.assign(K.s_ptr[N], R(0));
+  K.s_il.assign(K.s_ptr[N], 0);
   std::vector<int> fill(K.s_ptr.begin(), K.s_ptr.end() - 1);
   for (int p = 0; p < nproj; ++p)
     for (const Ent& e : byproj[p]) {
       const int k = fill[e.node]++;
-      K.s_proj[k] = p;
-      K.s_coef[k] = mk<T>(e.sr, e.si);
+      K.s_chi[k] = (R)e.chi;
+      K.s_il[k] = e.il;
     }
 }
 
@@ -213,14 +217,17 @@ static int upload_kb(KbTables<T>& D, const cyx_projector_desc* d, const double* 
   HostKb<T> K;
   build_kb<T>(d, W, kpt, N, K);
   D.nproj = (int)K.gamma.size();
+  D.nimgl = (int)K.il_proj.size();
   int rc = CYX_OK;
   if ((rc = upload_vec(D.g_ptr, K.g_ptr))) return rc;
   if ((rc = upload_vec(D.g_node, K.g_node))) return rc;
   if ((rc = upload_vec(D.g_coef, K.g_coef))) return rc;
   if ((rc = upload_vec(D.gamma, K.gamma))) return rc;
   if ((rc = upload_vec(D.s_ptr, K.s_ptr))) return rc;
-  if ((rc = upload_vec(D.s_proj, K.s_proj))) return rc;
-  if ((rc = upload_vec(D.s_coef, K.s_coef))) return rc;
+  if ((rc = upload_vec(D.s_chi, K.s_chi))) return rc;
+  if ((rc = upload_vec(D.s_il, K.s_il))) return rc;
+  if ((rc = upload_vec(D.il_phase, K.il_phase))) return rc;
+  if ((rc = upload_vec(D.il_proj, K.il_proj))) return rc;
   return CYX_OK;
 }
 
@@ -231,8 +238,10 @@ static void free_kb(KbTables<T>& D) {
   (void)hipFree(D.g_coef);
   (void)hipFree(D.gamma);
   (void)hipFree(D.s_ptr);
-  (void)hipFree(D.s_proj);
-  (void)hipFree(D.s_coef);
+  (void)hipFree(D.s_chi);
+  (void)hipFree(D.s_il);
+  (void)hipFree(D.il_phase);
+  (void)hipFree(D.il_proj);
   D = KbTables<T>{};
 }
 
@@ -662,6 +671,10 @@ int cyx_ham_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, const double kpt[3]
           set_error("rc_pos[%lld] = %d outside the grid", (long long)off, proj->rc_pos[off]);
           return fail(CYX_ERR_ARG);
         }
+    if (proj->n_image > 4096) {
+      set_error("too many projector images (%d > 4096)", proj->n_image);
+      return fail(CYX_ERR_ARG);
+    }
     if ((rc = upload_kb(h->kb_z, proj, W, kpt, N)) != CYX_OK) return fail(rc);
     if ((rc = upload_kb(h->kb_c, proj, W, kpt, N)) != CYX_OK) return fail(rc);
     if (!h->bloch) {
@@ -702,6 +715,7 @@ int cyx_ham_destroy(cyx_ham_t h) {
   (void)hipFree(h->kbC.p);
   (void)hipFree(h->small.p);
   (void)hipFree(h->sub.p);
+  (void)hipFree(h->red.p);
   delete h;
   return CYX_OK;
 }
@@ -808,12 +822,9 @@ int cyx_energy_terms(cyx_ham_t h, const double* rho, const double* vxc, const do
                      double rhotol, double out[3]) {
   CYX_CHECK(h && rho && vxc && phi && b && out, "null argument");
   const int nblk = 512;
-  DeviceBuf part;
   int rc;
-  if ((rc = ensure(part, 3 * (size_t)nblk * sizeof(double))) != CYX_OK) return rc;
-  rc = energy_terms(h->ctx->stream, h->N(), rho, vxc, phi, b, h->W, rhotol, (double*)part.p, nblk, out);
-  (void)hipFree(part.p);
-  return rc;
+  if ((rc = ensure(h->red, 3 * (size_t)nblk * sizeof(double))) != CYX_OK) return rc;
+  return energy_terms(h->ctx->stream, h->N(), rho, vxc, phi, b, h->W, rhotol, (double*)h->red.p, nblk, out);
 }
 
 int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* out) {

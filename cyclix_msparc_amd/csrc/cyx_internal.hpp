@@ -27,16 +27,21 @@ struct OpCoef {
 // KB tables in one arithmetic type T (complex phases for Bloch channels).
 template <class T>
 struct KbTables {
+  using R = typename traits<T>::real;
   int nproj = 0;
   // gather, sorted by projector: C[s][p] = gamma_p sum_e g[e] * Y[node[e]]
   int* g_ptr = nullptr;   // [nproj+1]
   int* g_node = nullptr;  // [nent]
   T* g_coef = nullptr;    // [nent]  chi * W * e^{+i k.d}
-  typename traits<T>::real* gamma = nullptr;  // [nproj]
-  // scatter, CSR by node: HY[n] += sum_e s[e] * C[s][proj[e]]
+  R* gamma = nullptr;     // [nproj]
+  // scatter, CSR by node: HY[n] += sum_e chi[e] * D[il[e]], where for one
+  // column D[il] = e^{-i k.d_img} * C[proj(il)] over (image, projector) pairs
   int* s_ptr = nullptr;   // [N+1]
-  int* s_proj = nullptr;  // [nent]
-  T* s_coef = nullptr;    // [nent]  chi * e^{-i k.d}
+  R* s_chi = nullptr;     // [nent]
+  int* s_il = nullptr;    // [nent] (image, projector) pair index
+  int nimgl = 0;
+  T* il_phase = nullptr;  // [nimgl] e^{-i k.d_img}
+  int* il_proj = nullptr; // [nimgl]
 };
 
 struct DeviceBuf {
@@ -109,6 +114,7 @@ struct cyx_ham_s {
   cyx::DeviceBuf kbC;      // C coefficients (ncol x nproj x 16 B)
   cyx::DeviceBuf small;    // Gram partial sums
   cyx::DeviceBuf sub;      // M, Hs, eigenvalues, solver work/info, column scales
+  cyx::DeviceBuf red;      // per-block partial sums of the energy and SCF-error reductions
   int64_t N() const { return (int64_t)g.nr * g.nt * g.nz; }
 };
 

@@ -45,12 +45,19 @@ struct StepParams {
   R s_r2, s_r1, s_z2, s_x;      // -1/2/dr^2, -1/2/dr, -1/2/dz^2, (-2 tau/dt)(-1/2/dz)
   R xin, dr;
   T ph_tl, ph_tr, ph_zl, ph_zr;
+  int trh;  // LDS row stride (elements): tr + halo + bank padding
   const int* s_ptr;
-  const int* s_proj;
-  const T* s_coef;
+  const R* s_chi;
+  const int* s_il;
+  const T* il_phase;
+  const int* il_proj;
+  int nimgl;
   const T* kbC;
   int nproj;
 };
+
+// KB entries per node held in registers (more are read from memory)
+constexpr int KB_SLOTS = 5;
 
 // non-temporal (streaming) element access: the X_prev stream and the result
 // are touched once, so they should not displace halo planes and Veff in L2
@@ -76,8 +83,9 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
 
-  const int TRH = P.tr + 2 * FDN;
+  const int TRH = P.trh;
   const int plane_sz = TRH * (P.nt + 2 * FDN);
+  T* Dsh = lds + 2 * plane_sz;  // per-column KB table D[il]
 
   int bid = blockIdx.x;
   if (P.xcd_remap) {
@@ -99,14 +107,21 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
   const int nrt = P.nr * P.nt;
   const int64_t colbase = (int64_t)s * P.ld;
   const T* __restrict__ src = P.in + colbase;
+  const int jr = j * P.nr + r;  // in-plane node offset
 
-  // per-radius scalars: 1/r, theta scale, diagonal (radial taps are
-  // cr2_p (x+ + x-) + (1/r) cr1_p (x+ - x-) with uniform cr2, cr1)
+  // per-radius scalars: (1/r) radial first-derivative factor, theta scale,
+  // diagonal; radial taps are s_r2 sum w2_p (x+ + x-) + rfac sum w1_p (x+ - x-)
   R rfac = 0, cth = 0, cdg = 0;
   if (rvalid) {
     rfac = P.s_r1 / (P.xin + R(r) * P.dr);
     cth = P.rcoef[r * RCOEF + 2 * FDN];
     cdg = P.rcoef[r * RCOEF + 2 * FDN + 1] - P.shift;
+  }
+
+  if constexpr (HAS_KB) {
+    // D[il] = e^{-i k.d_img} C[s][proj(il)] for this column
+    for (int t = tid; t < P.nimgl; t += blockDim.x)
+      Dsh[t] = cmul(P.il_phase[t], P.kbC[(int64_t)s * P.nproj + P.il_proj[t]]);
   }
 
   T q[QN];
@@ -142,13 +157,30 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
         xb = cmul(ph, xb);
       }
     }
-    ve = (rvalid && kk >= z0 && kk < z1) ? P.veff[(int64_t)kp * nrt + (int64_t)j * P.nr + r] : R(0);
+    ve = (rvalid && kk >= z0 && kk < z1) ? P.veff[(int64_t)kp * nrt + jr] : R(0);
   };
+
+  // KB pipeline.  At the start of step kk: (hold, cur, nxt) are the CSR row
+  // pointers of planes (kk, kk+1, kk+2) and echi/eil hold up to KB_SLOTS
+  // entries of plane kk.  The step consumes plane kk's entries, issues the
+  // entry loads of kk+1 and the row-pointer loads of kk+3.
+  int sp_hold0 = 0, sp_hold1 = 0;
+  int sp_cur0 = 0, sp_cur1 = 0;
+  int sp_nxt0 = 0, sp_nxt1 = 0;
+  R echi[KB_SLOTS];
+  int eil[KB_SLOTS];
+#pragma unroll
+  for (int e = 0; e < KB_SLOTS; ++e) {
+    echi[e] = R(0);
+    eil[e] = 0;
+  }
+  T pv_cur = zero<T>(), pv_nxt = zero<T>();
 
   T xa, xb;
   R ve;
   load_plane(kbeg, xa, xb, ve);
   int buf = 0;
+  if constexpr (HAS_KB) __syncthreads();  // Dsh ready
 
   for (int base = 0; base < nplanes; base += QN) {
 #pragma unroll
@@ -156,6 +188,7 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
       const int t = base + u;
       if (t < nplanes) {
         const int kk = kbeg + t;
+        const bool inner = kk >= z0 && kk < z1;
         // ---- stage plane kk in LDS (theta halo carries the Bloch phase)
         T* L = lds + buf * plane_sz;
         {
@@ -174,16 +207,50 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
           }
         }
         const R vcur = ve;
+        // ---- KB of plane kk into its queue slot; then issue the next loads
+        if constexpr (HAS_KB) {
+          if (inner && rvalid) {
+            T kb = zero<T>();
+            const int cnt = sp_hold1 - sp_hold0;
+#pragma unroll
+            for (int e = 0; e < KB_SLOTS; ++e)
+              if (e < cnt) kb = rfma(echi[e], Dsh[eil[e]], kb);
+            for (int e = sp_hold0 + KB_SLOTS; e < sp_hold1; ++e) kb = rfma(P.s_chi[e], Dsh[P.s_il[e]], kb);
+            q[u % QN] = add(q[u % QN], kb);
+          }
+          // entries of plane kk+1 (row pointers loaded one step ago)
+          sp_hold0 = sp_cur0;
+          sp_hold1 = sp_cur1;
+          {
+            const int cnt = sp_hold1 - sp_hold0;
+#pragma unroll
+            for (int e = 0; e < KB_SLOTS; ++e)
+              if (e < cnt) {
+                echi[e] = P.s_chi[sp_hold0 + e];
+                eil[e] = P.s_il[sp_hold0 + e];
+              }
+          }
+          // row pointers: (cur, nxt) become planes (kk+2, kk+3)
+          sp_cur0 = sp_nxt0;
+          sp_cur1 = sp_nxt1;
+          sp_nxt0 = sp_nxt1 = 0;
+          if (rvalid && kk + 3 >= z0 && kk + 3 < z1) {
+            const int64_t n3 = (int64_t)(kk + 3) * nrt + jr;
+            sp_nxt0 = P.s_ptr[n3];
+            sp_nxt1 = P.s_ptr[n3 + 1];
+          }
+        }
         __syncthreads();
-        // ---- prefetch: next plane, and prev of the plane emitted this step
+        // ---- prefetch: next plane, and X_prev of the plane emitted next step
         if (t + 1 < nplanes) load_plane(kk + 1, xa, xb, ve);
         const int ko = kk - FDN;
         const bool emit = (ko >= z0) && (ko < z1);
-        const int64_t nout = (int64_t)ko * nrt + (int64_t)j * P.nr + r;
-        T pv = zero<T>();
         if constexpr (HAS_PREV) {
-          if (emit && rvalid)
-            pv = P.nontemporal ? ntload(P.prev + colbase + nout) : P.prev[colbase + nout];
+          pv_cur = pv_nxt;
+          if (ko + 1 >= z0 && ko + 1 < z1 && rvalid) {
+            const T* pp = P.prev + colbase + (int64_t)(ko + 1) * nrt + jr;
+            pv_nxt = P.nontemporal ? ntload(pp) : *pp;
+          }
         }
         // ---- compute plane kk
         const int c = (j + FDN) * TRH + (i + FDN);
@@ -197,7 +264,7 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
           st = rfma(P.w2[p], add(xp, xm), st);
           if constexpr (HELICAL) tt = rfma(P.w1[p], sub(xp, xm), tt);
         }
-        if (kk >= z0 && kk < z1) {
+        if (inner) {
           T sr = zero<T>();
           T dr = zero<T>();
 #pragma unroll
@@ -233,18 +300,13 @@ __global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(
         {
           T& qo = q[(u + FDN + 1) % QN];
           if (emit && rvalid) {
-            T h = qo;
-            if constexpr (HAS_KB) {
-              const int e1 = P.s_ptr[nout + 1];
-              for (int e = P.s_ptr[nout]; e < e1; ++e)
-                h = cfma(P.s_coef[e], P.kbC[(int64_t)s * P.nproj + P.s_proj[e]], h);
-            }
-            T y = rmul(P.alpha, h);
-            if constexpr (HAS_PREV) y = rfma(-P.beta, pv, y);
+            T y = rmul(P.alpha, qo);
+            if constexpr (HAS_PREV) y = rfma(-P.beta, pv_cur, y);
+            T* po = P.out + colbase + (int64_t)ko * nrt + jr;
             if (P.nontemporal)
-              ntstore(P.out + colbase + nout, y);
+              ntstore(po, y);
             else
-              P.out[colbase + nout] = y;
+              *po = y;
           }
           qo = zero<T>();
         }
@@ -378,15 +440,21 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.ph_zl = phase_of<T>(h->ph[2][0], h->ph[2][1]);
   P.ph_zr = phase_of<T>(h->ph[3][0], h->ph[3][1]);
   P.s_ptr = K.s_ptr;
-  P.s_proj = K.s_proj;
-  P.s_coef = K.s_coef;
+  P.s_chi = K.s_chi;
+  P.s_il = K.s_il;
+  P.il_phase = K.il_phase;
+  P.il_proj = K.il_proj;
+  P.nimgl = K.nimgl;
   P.kbC = (const T*)h->kbC.p;
   P.nproj = K.nproj;
+  // LDS row stride: halo, then padding so that the next row starts on the
+  // next bank group (row pitch minus tile width a multiple of 256 B)
+  P.trh = P.tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
   const long nblk = (long)P.ntiles_r * P.nblocks_z * ncol;
   CYX_CHECK(nblk < (1L << 31), "grid too large");
   P.xcd_remap = (nblk % 8 == 0 && !(h->tune_flags & 2)) ? 1 : 0;
   const int nthr = P.tr * g.nt;
-  const size_t shm = 2 * (size_t)(P.tr + 2 * FDN) * (g.nt + 2 * FDN) * sizeof(T);
+  const size_t shm = (2 * (size_t)P.trh * (g.nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T);
   const bool hel = g.cell_typ == 4 || g.cell_typ == 5;
   const bool prev = prev_ != nullptr;
   (void)N;

@@ -77,6 +77,7 @@ typedef struct cyx_projector_desc {
 typedef struct cyx_ctx_s* cyx_ctx_t;
 typedef struct cyx_ham_s* cyx_ham_t;
 typedef struct cyx_poisson_s* cyx_poisson_t;
+typedef struct cyx_mixer_s* cyx_mixer_t;
 
 /* ------------------------------------------------------------ context */
 const char* cyx_last_error(void);
@@ -168,6 +169,23 @@ int cyx_lda_pw(cyx_ctx_t ctx, int64_t N, const double* rho, const double* phi, d
  * (src/evaluateTotalEnergy.m:27-52,72) with the handle's weights S.W. */
 int cyx_energy_terms(cyx_ham_t ham, const double* rho, const double* vxc, const double* phi, const double* b,
                      double rhotol, double out[3]);
+/* out = {Escc, ||vout - vin||^2, ||vout||^2} for potential mixing: the
+ * self-consistency correction sum((Veff - Veff_temp) .* rho .* W)
+ * (src/scf.m:160-162) and the two squared norms of the SCF error
+ * norm(Veff - Veff_temp) / norm(Veff) (src/scf.m:182-184). */
+int cyx_scf_terms(cyx_ham_t ham, const double* vout, const double* vin, const double* rho, double out[3]);
+
+/* Periodic-Pulay mixing of a length-N FP64 device vector
+ * (src/mixing.m:20-116 with MixingPrecond = 0, PulayRestartFreq = 0).
+ * create: history m in [1, 8] (S.MixingHistory), beta (S.MixingParameter),
+ * omega (S.MixingParameterSimple), p (S.PulayFrequency); x0 is the initial
+ * input iterate, stored as x_{k-1} with f_{k-1} = 0 and X = F = 0
+ * (src/scf.m:42-56).  step: x_out = mixing(g = output iterate, x = input
+ * iterate, iter), iter 1-based as count_SCF; x_out may alias g but not x. */
+int cyx_mixer_create(cyx_ctx_t ctx, int64_t N, int history, double beta, double omega, int pulay_frequency,
+                     const double* x0, cyx_mixer_t* out);
+int cyx_mixer_destroy(cyx_mixer_t mixer);
+int cyx_mixer_step(cyx_mixer_t mixer, const double* g, const double* x, double* x_out, int iter);
 
 /* ------------------------------------------------------------ Poisson */
 typedef struct cyx_aar_info {
