@@ -6,11 +6,14 @@ Workload: (10,0) carbon nanotube, 1/5 sector with helical twist 0.05 rad/Bohr,
 120 x 24 x 96 (N = 276,480), one complex Bloch channel (nu = 1,
 eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24, complex128 (FP64).
 
-One step = one steady-state SCF iteration on the device: CheFSI pass
+One step = one SCF iteration of src/scf.m on the device, at the SCF_tol of
+a 1e-5 Ha/atom energy target (src/initialization.m:516-552): CheFSI pass
 (filter + Rayleigh-Ritz + rotation + W-normalisation, src/eigSolver.m:98-124),
-occupations, electron density, AAR Poisson solve to 1e-8 relative residual
-(warm start, src/poissonSolve.m), LDA-PW Vxc, Veff update and the energy
-grid sums.  Inputs are resident in HBM when the timed region starts.
+occupations, electron density, AAR Poisson solve to poisson_tol (warm
+start, src/poissonSolve.m), LDA-PW Vxc and Veff, the energy grid sums and
+Periodic-Pulay potential mixing.  Inputs are resident in HBM when the timed
+region starts.  The workload's standalone Poisson solve (atom-Gaussian
+density, cold start, 1e-8 relative residual) is timed separately.
 
 Prints one JSON line (metric, value in ms per SCF step, roofline of the
 dominant kernel from live HIP-event timing, CPU baseline from the oracle).
@@ -49,18 +52,6 @@ def parse():
     return p.parse_args()
 
 
-def fermi_occ(ev, nelec, bet):
-    """Gaussian-smearing Fermi level and occupations (src/occupations.m:3-15)."""
-    from scipy.optimize import brentq
-    from scipy.special import erf
-
-    def f(lf):
-        return 2.0 * np.sum(0.5 * (1 - erf(bet * (ev - lf)))) - nelec
-
-    lf = brentq(f, ev.min() - 10, ev.max() + 10, xtol=1e-14)
-    return lf, 0.5 * (1 - erf(bet * (ev - lf)))
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -69,64 +60,47 @@ def main():
         sys.exit(2)
     import torch
 
-    from cyclix_msparc_amd import _native as nat
-    from cyclix_msparc_amd.api import Device, Hamiltonian, Poisson, lda_pw
+    from cyclix_msparc_amd.api import Device, Poisson
+    from cyclix_msparc_amd.scf import GroundState, SCFOptions
     from cyclix_msparc_amd.system import make_system
 
-    dtype_code = nat.DTYPE_NAMES[args.dtype]
     bsz = 16 if args.dtype == "c128" else 8
     sysm = make_system(args.workload)
     g = sysm.grid
     N, ns, npl = g.N, sysm.nstates, sysm.npl
-    proj = sysm.projectors()
-    pc = sysm.pseudocharge()
     dev = Device(0)
     tdev = dev.torch_device
-    b = torch.from_numpy(pc["b"]).to(tdev)
-    rho = torch.from_numpy(pc["rho_at"]).to(tdev)
-    poisson = Poisson(dev, g)
-    phi, info0 = poisson.solve(rho, b, None, 1e-8, 1000)
-    vxc, veff = lda_pw(dev, rho, phi)
-    ham = Hamiltonian(dev, sysm, sysm.kpt, veff.cpu().numpy(), proj)
+    # SCF_tol for 1e-5 Ha/atom (src/initialization.m:516-545), poisson_tol = 0.01 SCF_tol;
+    # MINIT_SCF keeps the loop running through warmup + timed iterations
+    opts = SCFOptions(target_energy_accuracy=1e-5, MINIT_SCF=args.warmup + args.steps,
+                      MAXIT_SCF=max(100, args.warmup + args.steps), MAXIT_POISSON=args.aar_maxit)
+    gs = GroundState(dev, sysm, options=opts, precision="fp64" if args.dtype == "c128" else "fp32")
     rng = np.random.default_rng(0)
     X = rng.standard_normal((ns, N)) + 1j * rng.standard_normal((ns, N))
-    tdt = torch.complex128 if args.dtype == "c128" else torch.complex64
-    psi = torch.from_numpy(X).to(tdev).to(tdt).contiguous()
-    work = torch.empty_like(psi)
-    # first SCF iteration: Lanczos bounds (src/eigSolver.m:47-63, 88-94)
-    lo, hi = ham.lanczos_bounds(np.random.default_rng(2).random(N), 300, 1e-2)
-    bup, a0 = 1.01 * hi, lo - 0.1
-    cut = 0.5 * (bup + a0)
-    bet = 27.211386245988 / 0.2
-    nelec = sysm.nelectron
-    state = dict(cut=cut, a0=a0, phi=phi, ev=None)
-
-    def scf_step():
-        ev = ham.eig_pass(psi, npl, state["cut"], bup, state["a0"], work=work)
-        lf, occ = fermi_occ(ev, nelec, bet)
-        rho.zero_()
-        ham.density_accumulate(rho, psi, occ, 1.0)
-        phi_new, info = poisson.solve(rho, b, state["phi"], 1e-8, args.aar_maxit)
-        vxc_new, veff_new = lda_pw(dev, rho, phi_new)
-        terms = ham.energy_terms(rho, vxc_new, phi_new, b)
-        ham.set_veff(veff_new)
-        state.update(phi=phi_new, ev=ev, a0=float(ev.min()), cut=float(ev.max()) + 0.10, info=info, terms=terms)
-
+    if args.dtype == "c64":
+        X = X.astype(np.complex64)
+    gs.init_psi([X])
+    del X
+    it = gs.iterations()
+    # warmup: the first pass with its Lanczos bounds, the Chebyshev cycles
+    # before rhoTrigger and `warmup` SCF iterations (src/scf.m:111-125)
     for _ in range(args.warmup):
-        scf_step()
+        next(it)
     dev.synchronize()
     torch.cuda.synchronize()
     dev.set_timing(True)
     dev.reset_timing()
+    recs = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        scf_step()
+        recs.append(next(it))
     dev.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / args.steps
     tm = dev.timing()
     dev.set_timing(False)
+    it.close()
 
     # roofline of the dominant kernel: fused stencil + three-term step
     # algorithmic bytes per launch = read Y + read X_prev + write Y_new (ns
@@ -135,15 +109,27 @@ def main():
     bytes_launch = 3.0 * N * ns * bsz + 8.0 * N
     avg_ms = st_ms / max(st_n, 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    # whole filter (gather + stencil, m steps): B_filter = (3m-1) N ns b + m N 8
-    filt_ms = (tm["stencil"][0] + tm["kb_gather"][0] + tm["h_apply"][0]) / args.steps
+    # whole filter: B_filter = (3m-1) N ns b + m N 8; its time is the m-1
+    # stencil launches, the first-step H-apply and the m KB gathers of the
+    # filter (Rayleigh-Ritz adds one H-apply and one gather per pass)
     ha_ms, ha_n = tm["h_apply"]
-    # h_apply launches per step: first filter step + H psi of Rayleigh-Ritz;
-    # count only the filter's share
-    filt_ms_only = (tm["stencil"][0] / args.steps + (ha_ms / max(ha_n, 1)) +
-                    tm["kb_gather"][0] / args.steps * npl / (npl + 1))
+    kb_ms, kb_n = tm["kb_gather"]
+    filt_ms_only = (st_ms + ha_ms / 2 + kb_ms * npl / (npl + 1)) / args.steps
     b_filter = (3 * npl - 1) * N * ns * bsz + npl * N * 8.0
     filter_gbs = b_filter / (filt_ms_only * 1e-3) / 1e9
+
+    # the Poisson solve of the workload on its own: rho = sum of atom-centred
+    # Gaussians, cold start, 1e-8 relative residual (src/poissonSolve.m)
+    pc = sysm.pseudocharge()
+    b = torch.from_numpy(pc["b"]).to(tdev)
+    rho_at = torch.from_numpy(pc["rho_at"]).to(tdev)
+    poisson = Poisson(dev, g)
+    poisson.solve(rho_at, b, None, 1e-8, args.aar_maxit)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, info8 = poisson.solve(rho_at, b, None, 1e-8, args.aar_maxit)
+    torch.cuda.synchronize()
+    aar_ms = (time.perf_counter() - t) * 1e3
 
     if args.calib_copy:
         src = torch.ones(1 << 26, dtype=torch.float64, device=tdev)
@@ -151,9 +137,13 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
 
+    a0, bup, cut = gs.bounds
+    aar_its = float(np.mean([r.aar_iterations for r in recs]))
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sysm, proj, pc, veff.cpu().numpy(), state, ms_step, args, tm)
+        cpu = cpu_baseline(sysm, sysm.projectors(), pc, gs.veff.cpu().numpy(),
+                           dict(cut=float(cut[0]), a0=float(a0[0]), bup=float(bup[0]), aar_its=aar_its),
+                           ms_step, args)
 
     phases = {k: round(v[0] / args.steps, 3) for k, v in tm.items()}
     out = {
@@ -171,7 +161,9 @@ def main():
         "data": "synthetic (seeded psi0 ~ N(0,1); synthetic Gaussian pseudopotential)",
         "config": {"workload": f"{args.workload}: (10,0) CNT 1/5 sector, twist 0.05, 8 atoms, grid "
                                f"{g.nr}x{g.nt}x{g.nz} (N={N}), {ns} states, Chebyshev degree {npl}, "
-                               "1 Bloch channel, AAR Poisson tol 1e-8",
+                               "1 Bloch channel; SCF iterations with Periodic-Pulay potential mixing, "
+                               f"SCF_tol {gs.opt.SCF_tol:.3e} (1e-5 Ha/atom), AAR Poisson tol "
+                               f"{gs.opt.poisson_tol:.3e}",
                    "path": "FP64 GPU" if args.dtype == "c128" else "mixed (complex64 filter)"},
         "roofline": {"bound": "hbm", "kernel": "stencil_step_kernel (fused H-apply + Chebyshev step)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -180,13 +172,16 @@ def main():
         "filter": {"ms": round(filt_ms_only, 3), "GB/s": round(filter_gbs, 1),
                    "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4)},
         "phase_ms_per_step": phases,
-        "aar_iterations": int(state["info"].iterations),
+        "scf": {"err": [round(r.err, 6) for r in recs], "Eatom": [round(r.Etotal / sysm.n_atm, 8) for r in recs],
+                "aar_iterations": [r.aar_iterations for r in recs]},
+        "poisson_1e-8": {"ms": round(aar_ms, 2), "iterations": int(info8.iterations),
+                         "converged": bool(info8.converged)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
-def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args, tm):
+def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args):
     """The plain-C FP64 oracle (OpenMP) on a bounded sample of the same SCF
     step, scaled to one step: the degree-npl filter on `cpu-sample-cols`
     columns (scaled by ns / cols), the three Rayleigh-Ritz GEMMs with numpy
@@ -203,7 +198,7 @@ def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args, tm):
     rng = np.random.default_rng(0)
     X = rng.standard_normal((N, c)) + 1j * rng.standard_normal((N, c))
     t = time.perf_counter()
-    Cc.chebyshev_filter(X, npl, state["cut"], 1.0e3, state["a0"])
+    Cc.chebyshev_filter(X, npl, state["cut"], state["bup"], state["a0"])
     t_filter = (time.perf_counter() - t) * ns / c
     n_rr = N // 16
     A = rng.standard_normal((n_rr, ns)) + 1j * rng.standard_normal((n_rr, ns))
@@ -215,7 +210,7 @@ def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args, tm):
     f = O.poisson_rhs(pc["rho_at"], pc["b"])
     t = time.perf_counter()
     Cc.aar_jacobi(f, None, 1e-30, 20)
-    t_aar = (time.perf_counter() - t) / 20 * max(int(state["info"].iterations), 1)
+    t_aar = (time.perf_counter() - t) / 20 * max(state["aar_its"], 1.0)
     total = t_filter + t_rr + t_aar
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": round(total * 1e3, 1), "unit": "ms/SCF-step", "cores": cores, "kind": "port",

@@ -382,3 +382,15 @@ def small_system(nr=14, nt=16, nz=14, twist=0.0, kpt=(0.0, 0.0, 0.0), nstates=8,
     g = GridSpec(nr, nt, nz, xin, dr, L2 / nt, L3 / nz, twist, 6, cell)
     sp = Species(rc_nl=1.6, rb=2.4, sigma_nl=0.5)
     return CyclixSystem(g, atoms, sp, nstates=nstates, npl=npl, kpt=kpt, name="small")
+
+
+def scf_system(nr=28, nt=12, nz=16, vacuum=6.0, twist=0.0, nstates=12, npl=12):
+    """(6,0) tube, 1/6 sector, 4 C, on a coarse grid (default 28 x 12 x 16,
+    N = 5376) with 6 Bohr of radial vacuum on each side: a whole SCF runs
+    in seconds on the CPU.  The pseudocharge integrates to -16 within 1e-5."""
+    atoms, R, L2, L3 = zigzag_sector_atoms(6, 4, twist=twist)
+    xin = max(R - vacuum, 1.5)
+    xout = R + vacuum
+    cell = 5 if twist != 0.0 else 3
+    g = GridSpec(nr, nt, nz, xin, (xout - xin) / (nr - 1), L2 / nt, L3 / nz, twist, 6, cell)
+    return CyclixSystem(g, atoms, Species(), nstates=nstates, npl=npl, kpt=(0.0, 0.0, 0.0), name="cnt6_scf")

@@ -657,3 +657,72 @@ def sternheimer_apply(ham: Hamiltonian, lam, psi_v, X):
     HX = HX - X * np.asarray(lam)[None, :]
     HX = HX + psi_v @ (psi_v.conj().T @ (ham.W[:, None] * X))
     return HX
+
+
+# --------------------------------------------------------------------------
+# SCF loop
+# --------------------------------------------------------------------------
+def scf(g: Grid, W, atoms, channels, wkpt, bcharge, rho0, Eself, Ecorr, nelec, n_atom, psi0, npl, SCF_tol,
+        poisson_tol, maxit_scf=100, minit_scf=3, rho_trigger=4, hist=7, beta=0.3, omega=None, pfreq=1,
+        bet=27.211386245988 / 0.2, kind=1, tol_lanczos=1e-2, lanczos_maxit=300, maxit_poisson=1000,
+        rhotol=1e-14):
+    """``src/scf.m:32-306`` for nspin = 1, potential mixing, BC = 3 (no mean
+    shift of Veff) and the first relaxation step.  psi0: list of (N, Nev)
+    blocks, one per channel.  Returns a dict with the converged energies,
+    eigenvalues and the per-iteration record."""
+    phi, _, _ = poisson_solve(g, rho0, bcharge, poisson_tol, None, maxit_poisson)
+    vxc = lda_pw_vxc(rho0, rhotol)
+    veff = phi + vxc                                        # src/scf.m:39
+    mix = PeriodicPulay(g.N, hist, beta, omega, pfreq)
+    veff_in = veff.copy()
+    psis = [np.array(p, copy=True) for p in psi0]
+    nch = len(channels)
+    Nev = psis[0].shape[1]
+    EigVal = np.zeros((Nev, nch))
+    bup, a0, cut = np.zeros(nch), np.zeros(nch), np.zeros(nch)
+    Wm = metric_weights(g)
+    count, count_scf, err = 1, 1, 100.0
+    history = []
+    Etot, lam_f, occs = 0.0, 0.0, None
+    min_scf = min(minit_scf, maxit_scf)
+    while (err > SCF_tol and count_scf <= maxit_scf) or count_scf <= min_scf:
+        for ks, kpt in enumerate(channels):                 # src/eigSolver.m:26-124
+            ham = Hamiltonian(g, tuple(kpt), veff, W, atoms)
+            if count == 1:
+                v0 = np.random.default_rng(ks + 1).random(g.N)
+                lo, hi = lanczos_bounds(ham.apply, v0, lanczos_maxit, tol_lanczos, Wm)
+                bup[ks] = 1.01 * hi
+                a0[ks] = lo - 0.1
+                cut[ks] = 0.5 * (bup[ks] + a0[ks])
+            else:
+                if count > rho_trigger:
+                    a0[ks] = np.min(EigVal[:, ks])
+                cut[ks] = np.max(EigVal[:, ks]) + 0.10
+            ev, psis[ks] = eig_solver_pass(ham, psis[ks], npl, cut[ks], bup[ks], a0[ks])
+            EigVal[:, ks] = ev
+        evs = [EigVal[:, ks] for ks in range(nch)]
+        lam_f = fermi_level(evs, wkpt, bet, nelec, 2.0, kind)  # src/occupations.m
+        occs = [smearing_occ(e, lam_f, bet, kind) for e in evs]
+        if count >= rho_trigger:
+            rho = electron_density(psis, occs, wkpt)
+            phi, _, it_p = poisson_solve(g, rho, bcharge, poisson_tol, phi, maxit_poisson)
+            vxc = lda_pw_vxc(rho, rhotol)
+            veff_out = phi + vxc
+            Etot, Eband, Exc, Exc_dc, Eelec_dc, Eent = total_energy(evs, occs, wkpt, lam_f, bet, rho, vxc, phi,
+                                                                   bcharge, W, Eself, Ecorr, kind, rhotol)
+            Escc = float(np.sum((veff_out - veff_in) * rho * W))     # src/scf.m:160-162
+            Etot += Escc
+            err = np.linalg.norm(veff_out - veff_in) / np.linalg.norm(veff_out)   # src/scf.m:184
+            history.append(dict(count_SCF=count_scf, Etotal=Etot, Eband=Eband, Exc=Exc, Exc_dc=Exc_dc,
+                                Eelec_dc=Eelec_dc, Eent=Eent, Escc=Escc, lambda_f=lam_f, err=err,
+                                aar_iterations=it_p))
+            if err > SCF_tol or count_scf < minit_scf:
+                veff = mix(veff_out, veff_in, count_scf)    # src/scf.m:193-209
+                veff_in = veff.copy()
+            else:
+                veff = veff_out
+            count_scf += 1
+        count += 1
+    converged = not (count_scf == maxit_scf + 1 and err > SCF_tol)
+    return dict(converged=converged, count_SCF=count_scf - 1, Etotal=Etot, Eatom=Etot / n_atom, lambda_f=lam_f,
+                EigVal=EigVal, occ=np.stack(occs, 1), history=history, psi=psis, rho=rho, phi=phi, veff=veff)

@@ -169,3 +169,64 @@ def test_sternheimer_operator_matches_dense():
     Hd = H.dense()
     ref = np.stack([(Hd - lam[k] * np.eye(N) + psi_v @ psi_v.conj().T @ np.diag(sysm.W)) @ X[:, k] for k in range(2)], 1)
     assert np.allclose(O.sternheimer_apply(H, lam, psi_v, X), ref, rtol=1e-10, atol=1e-10)
+
+
+# ----------------------------------------------------------------- mixing
+def _affine_map(n, seed, contraction=0.6):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    M = Q @ np.diag(np.linspace(-contraction, contraction, n)) @ Q.T
+    c = rng.standard_normal(n)
+    return M, c, np.linalg.solve(np.eye(n) - M, c)
+
+
+def test_pulay_first_step_is_simple_mixing():
+    M, c, _ = _affine_map(6, 1)
+    x0 = np.zeros(6)
+    mix = O.PeriodicPulay(6, hist=7, beta=0.3)
+    x1 = mix(M @ x0 + c, x0, 1)
+    assert np.allclose(x1, x0 + 0.3 * (M @ x0 + c - x0), rtol=0, atol=1e-15)
+
+
+def test_pulay_terminates_on_affine_fixed_point():
+    """With history >= n, Anderson mixing of an affine map is a Krylov method
+    and reaches the fixed point within n + 2 iterations; a dropped X*Gamma or
+    F*Gamma term or a wrong history index breaks the finite termination."""
+    n = 5
+    M, c, xs = _affine_map(n, 2)
+    x = np.zeros(n)
+    mix = O.PeriodicPulay(n, hist=7, beta=0.3)
+    for it in range(1, n + 3):
+        x = mix(M @ x + c, x, it)
+    assert np.linalg.norm(x - xs) < 1e-9 * np.linalg.norm(xs)
+
+
+def test_pulay_weighted_residual_is_least_squares_optimal():
+    """f_w = f_k - F Gamma is orthogonal to the columns of F."""
+    rng = np.random.default_rng(3)
+    N, m = 40, 7
+    F = rng.standard_normal((N, m))
+    F[:, 5:] = 0.0      # unfilled history columns
+    X = rng.standard_normal((N, m))
+    f = rng.standard_normal(N)
+    x = rng.standard_normal(N)
+    xw, fw = O.anderson_wtd_avg(x, f, X, F)
+    assert np.max(np.abs(F.T @ fw)) < 1e-12
+    Gamma = np.linalg.lstsq(F[:, :5], f, rcond=None)[0]
+    assert np.allclose(xw, x - X[:, :5] @ Gamma, atol=1e-12)
+
+
+def test_pulay_frequency_alternates_simple_and_anderson():
+    """With p = 2 odd iterations are plain omega-mixing steps; history still updates."""
+    M, c, _ = _affine_map(4, 4)
+    x = np.ones(4)
+    mix = O.PeriodicPulay(4, hist=3, beta=0.5, omega=0.1, pfreq=2)
+    x1 = mix(M @ x + c, x, 1)
+    assert np.allclose(x1, x + 0.1 * (M @ x + c - x), atol=1e-15)
+    x2 = mix(M @ x1 + c, x1, 2)
+    f1, f2 = M @ x + c - x, M @ x1 + c - x1
+    dX, dF = x1 - x, f2 - f1
+    gam = (dF @ f2) / (dF @ dF)
+    assert np.allclose(x2, x1 - dX * gam + 0.5 * (f2 - dF * gam), atol=1e-13)
+    x3 = mix(M @ x2 + c, x2, 3)
+    assert np.allclose(x3, x2 + 0.1 * (M @ x2 + c - x2), atol=1e-15)
