@@ -25,6 +25,7 @@ EXPORTS = (
     "cyx_rayleigh_ritz", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
+    "cyx_sternheimer_apply", "cyx_sternheimer_solve",
 )
 
 
@@ -55,6 +56,10 @@ class Tuning(C.Structure):
 
 class AarInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double)]
+
+
+class SternInfo(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("n_converged", C.c_int32), ("max_relres", C.c_double)]
 
 
 _lib = None
@@ -102,6 +107,9 @@ def load():
         "cyx_mixer_create": (i32, [vp, i64, i32, dbl, dbl, i32, vp, C.POINTER(vp)]),
         "cyx_mixer_destroy": (i32, [vp]),
         "cyx_mixer_step": (i32, [vp, vp, vp, vp, i32]),
+        "cyx_sternheimer_apply": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32]),
+        "cyx_sternheimer_solve": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32, i32, dbl, i32, i32,
+                                        C.POINTER(SternInfo), C.POINTER(C.c_int32), pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

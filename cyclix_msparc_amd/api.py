@@ -239,6 +239,50 @@ class Hamiltonian:
                                          C.c_void_p(rho.data_ptr()), nat.dptr(out)))
         return tuple(out)
 
+    def sternheimer_apply(self, X, lam, psi_v=None):
+        """Y_j = (H - lam_j) X_j + psi_v psi_v^H (W .* X_j)  (``src/phonon.m:2139-2157``);
+        X: (ncol, N) block, psi_v: (nv, N) block of the same dtype or None."""
+        _check_block(X, self.N)
+        lam = np.ascontiguousarray(np.asarray(lam, dtype=float))
+        if lam.size != X.shape[0]:
+            raise ValueError("one shift per column")
+        nv = 0 if psi_v is None else psi_v.shape[0]
+        if psi_v is not None:
+            _check_block(psi_v, self.N)
+            if psi_v.dtype != X.dtype:
+                raise ValueError("psi_v and X must share a dtype")
+        Y = torch.empty_like(X)
+        nat.check(self.lib.cyx_sternheimer_apply(self.h, _dtype_code(X), C.c_void_p(X.data_ptr()),
+                                                 C.c_void_p(Y.data_ptr()), X.shape[0], self.N, nat.dptr(lam),
+                                                 C.c_void_p(psi_v.data_ptr()) if nv else None, self.N, nv))
+        return Y
+
+    def sternheimer_solve(self, B, lam, psi_v=None, X0=None, tol=1e-8, max_iter=3000, precond=True,
+                          raise_on_noconv=False):
+        """Batched BiCGSTAB on the Sternheimer systems, one per column
+        (``src/phonon.m:1516-1521``).  Returns (X, info, iterations, relres)."""
+        _check_block(B, self.N)
+        lam = np.ascontiguousarray(np.asarray(lam, dtype=float))
+        if lam.size != B.shape[0]:
+            raise ValueError("one shift per column")
+        nv = 0 if psi_v is None else psi_v.shape[0]
+        if psi_v is not None:
+            _check_block(psi_v, self.N)
+            if psi_v.dtype != B.dtype:
+                raise ValueError("psi_v and B must share a dtype")
+        X = X0.clone().contiguous() if X0 is not None else torch.empty_like(B)
+        info = nat.SternInfo()
+        its = np.zeros(B.shape[0], dtype=np.int32)
+        rel = np.zeros(B.shape[0])
+        allow = () if raise_on_noconv else (nat.CYX_ERR_NOCONV,)
+        nat.check(self.lib.cyx_sternheimer_solve(self.h, _dtype_code(B), C.c_void_p(B.data_ptr()),
+                                                 C.c_void_p(X.data_ptr()), B.shape[0], self.N, nat.dptr(lam),
+                                                 C.c_void_p(psi_v.data_ptr()) if nv else None, self.N, nv,
+                                                 1 if X0 is not None else 0, float(tol), int(max_iter),
+                                                 1 if precond else 0, C.byref(info), nat.iptr(its),
+                                                 nat.dptr(rel)), allow=allow)
+        return X, info, its, rel
+
     def lanczos_bounds(self, v0, maxit=300, tol=1e-3):
         v0 = np.ascontiguousarray(np.asarray(v0, dtype=float))
         lo, hi = C.c_double(), C.c_double()

@@ -716,6 +716,7 @@ int cyx_ham_destroy(cyx_ham_t h) {
   (void)hipFree(h->small.p);
   (void)hipFree(h->sub.p);
   (void)hipFree(h->red.p);
+  (void)hipFree(h->stern.p);
   delete h;
   return CYX_OK;
 }

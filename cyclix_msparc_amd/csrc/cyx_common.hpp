@@ -108,6 +108,40 @@ __device__ __forceinline__ R absq(cplx<R> a) {
   return a.x * a.x + a.y * a.y;
 }
 
+// FP64 accumulator type of T
+template <class T>
+struct acc_of {
+  using type = double;
+};
+template <class R>
+struct acc_of<cplx<R>> {
+  using type = cplx<double>;
+};
+
+__device__ __forceinline__ double widen(double a) { return a; }
+__device__ __forceinline__ double widen(float a) { return (double)a; }
+__device__ __forceinline__ cplx<double> widen(cplx<double> a) { return a; }
+__device__ __forceinline__ cplx<double> widen(cplx<float> a) { return cplx<double>{(double)a.x, (double)a.y}; }
+
+template <class T>
+__device__ __forceinline__ T narrow(typename acc_of<T>::type a);
+template <>
+__device__ __forceinline__ double narrow<double>(double a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ float narrow<float>(double a) {
+  return (float)a;
+}
+template <>
+__device__ __forceinline__ cplx<double> narrow<cplx<double>>(cplx<double> a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ cplx<float> narrow<cplx<float>>(cplx<double> a) {
+  return cplx<float>{(float)a.x, (float)a.y};
+}
+
 // host-side phase e^{i phi} in type T (real types: only phi == 0 is valid)
 template <class T>
 inline T phase_of(double c, double s);

@@ -6,48 +6,14 @@
 
 namespace cyx {
 
-// FP64 accumulator type of T
-template <class T>
-struct acc_of {
-  using type = double;
-};
-template <class R>
-struct acc_of<cplx<R>> {
-  using type = cplx<double>;
-};
-
-__device__ __forceinline__ double widen(double a) { return a; }
-__device__ __forceinline__ double widen(float a) { return (double)a; }
-__device__ __forceinline__ cplx<double> widen(cplx<double> a) { return a; }
-__device__ __forceinline__ cplx<double> widen(cplx<float> a) { return cplx<double>{(double)a.x, (double)a.y}; }
-
-template <class T>
-__device__ __forceinline__ T narrow(typename acc_of<T>::type a);
-template <>
-__device__ __forceinline__ double narrow<double>(double a) {
-  return a;
-}
-template <>
-__device__ __forceinline__ float narrow<float>(double a) {
-  return (float)a;
-}
-template <>
-__device__ __forceinline__ cplx<double> narrow<cplx<double>>(cplx<double> a) {
-  return a;
-}
-template <>
-__device__ __forceinline__ cplx<float> narrow<cplx<float>>(cplx<double> a) {
-  return cplx<float>{(float)a.x, (float)a.y};
-}
-
 constexpr int GT = 64;  // output tile
 constexpr int GK = 16;  // k chunk
 
-// Partial[kb][a][b] = sum_{n in chunk kb} conj(A[n,a]) W[n] B[n,b]  (column-major ns x ns per partial)
+// Partial[kb][a][b] = sum_{n in chunk kb} conj(A[n,a]) W[n] B[n,b]  (column-major na x nb per partial)
 template <class T>
-__global__ void __launch_bounds__(256) gram_partial_kernel(const T* __restrict__ A, const T* __restrict__ B,
-                                                           const double* __restrict__ W, int64_t N, int64_t ld,
-                                                           int ns, int64_t krange,
+__global__ void __launch_bounds__(256) gram_partial_kernel(const T* __restrict__ A, int64_t lda, int na,
+                                                           const T* __restrict__ B, int64_t ldb, int nb,
+                                                           const double* __restrict__ W, int64_t N, int64_t krange,
                                                            typename acc_of<T>::type* __restrict__ part) {
   using D = typename acc_of<T>::type;
   __shared__ D sa[GK][GT + 1];
@@ -73,9 +39,9 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const T* __restrict__
       D va{}, vb{};
       if (n < k1) {
         const double w = W[n];
-        if (a0 + col < ns) va = widen(A[(int64_t)(a0 + col) * ld + n]);
-        if (b0 + col < ns) {
-          const D bb = widen(B[(int64_t)(b0 + col) * ld + n]);
+        if (a0 + col < na) va = widen(A[(int64_t)(a0 + col) * lda + n]);
+        if (b0 + col < nb) {
+          const D bb = widen(B[(int64_t)(b0 + col) * ldb + n]);
           vb = rmul(w, bb);
         }
       }
@@ -97,13 +63,13 @@ __global__ void __launch_bounds__(256) gram_partial_kernel(const T* __restrict__
     }
     __syncthreads();
   }
-  D* out = part + (int64_t)kb * ns * ns;
+  D* out = part + (int64_t)kb * na * nb;
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
       const int a = a0 + tx + 16 * x, b = b0 + ty + 16 * y;
-      if (a < ns && b < ns) out[(int64_t)b * ns + a] = acc[x][y];
+      if (a < na && b < nb) out[(int64_t)b * na + a] = acc[x][y];
     }
 }
 
@@ -224,40 +190,46 @@ __global__ void hermitize_kernel(D* M, int ns) {
 
 // ---------------------------------------------------------------- host side
 template <class T>
-static int gram_impl(cyx_ham_s* h, const void* A, const void* B, int ns, int64_t ld, void* M_out) {
+static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                     const double* W, void* M_out) {
   using D = typename acc_of<T>::type;
   const int64_t N = h->N();
-  const int nt = ceil_div(ns, GT);
-  int kparts = ceil_div(2048, nt * nt);
+  const int ta = ceil_div(na, GT), tb = ceil_div(nb, GT);
+  int kparts = ceil_div(2048, ta * tb);
   int64_t krange = (N + kparts - 1) / kparts;
   krange = ((krange + GK - 1) / GK) * GK;
   kparts = (int)((N + krange - 1) / krange);
-  const size_t pbytes = (size_t)kparts * ns * ns * sizeof(D);
+  const size_t pbytes = (size_t)kparts * na * nb * sizeof(D);
   CYX_CHECK(ensure(h->small, pbytes) == CYX_OK, "partials allocation failed");
   hipStream_t st = h->ctx->stream;
   PhaseTimer tm(h->ctx, PH_GRAM);
-  hipLaunchKernelGGL(gram_partial_kernel<T>, dim3(nt, nt, kparts), dim3(256), 0, st, (const T*)A, (const T*)B, h->Wm, N,
-                     ld, ns, krange, (D*)h->small.p);
+  hipLaunchKernelGGL(gram_partial_kernel<T>, dim3(ta, tb, kparts), dim3(256), 0, st, (const T*)A, lda, na,
+                     (const T*)B, ldb, nb, W, N, krange, (D*)h->small.p);
   CYX_HIP(hipGetLastError());
-  const int64_t len = (int64_t)ns * ns;
+  const int64_t len = (int64_t)na * nb;
   hipLaunchKernelGGL(reduce_partials_kernel<D>, dim3(ceil_div(len, 256)), dim3(256), 0, st, (const D*)h->small.p,
                      kparts, len, (D*)M_out);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
 
-int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out) {
+int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+              const double* W, void* M_out) {
   switch (dtype) {
     case CYX_F64:
-      return gram_impl<double>(h, A, B, ns, ld, M_out);
+      return gram_impl<double>(h, A, lda, na, B, ldb, nb, W, M_out);
     case CYX_F32:
-      return gram_impl<float>(h, A, B, ns, ld, M_out);
+      return gram_impl<float>(h, A, lda, na, B, ldb, nb, W, M_out);
     case CYX_C128:
-      return gram_impl<cplx<double>>(h, A, B, ns, ld, M_out);
+      return gram_impl<cplx<double>>(h, A, lda, na, B, ldb, nb, W, M_out);
     case CYX_C64:
-      return gram_impl<cplx<float>>(h, A, B, ns, ld, M_out);
+      return gram_impl<cplx<float>>(h, A, lda, na, B, ldb, nb, W, M_out);
   }
   return CYX_ERR_ARG;
+}
+
+int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out) {
+  return gram_rect(h, dtype, A, ld, ns, B, ld, ns, h->Wm, M_out);
 }
 
 template <class T>

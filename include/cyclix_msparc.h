@@ -187,6 +187,34 @@ int cyx_mixer_create(cyx_ctx_t ctx, int64_t N, int history, double beta, double 
 int cyx_mixer_destroy(cyx_mixer_t mixer);
 int cyx_mixer_step(cyx_mixer_t mixer, const double* g, const double* x, double* x_out, int iter);
 
+/* ------------------------------------------------------------ Sternheimer (DFPT) */
+/* Y_j = (H - lam[j]) X_j + psi_v (psi_v^H (W .* X_j)) for the ncol columns of
+ * X: the operator H_times_x of src/phonon.m:2139-2157 (one shift per column,
+ * H the handle's channel operator including the KB term, W the handle's
+ * weights).  X, Y: device blocks (ncol x ld), psi_v: device block of nv
+ * occupied states (nv x ldv; nv = 0 drops the projector term); lam: host,
+ * ncol doubles. */
+int cyx_sternheimer_apply(cyx_ham_t ham, int dtype, const void* X, void* Y, int ncol, int64_t ld, const double* lam,
+                          const void* psi_v, int64_t ldv, int nv);
+
+typedef struct cyx_stern_info {
+  int32_t max_iterations; /* largest per-column iteration count */
+  int32_t n_converged;    /* columns with ||b - A x|| <= tol ||b|| (recursive residual) */
+  double max_relres;      /* largest per-column relative residual */
+} cyx_stern_info;
+
+/* X_j = bicgstab(A_j, B_j, tol, max_iter, M, X_j) for every column at once,
+ * A_j the operator above: replaces the per-state bicgstab loop at
+ * src/phonon.m:1516-1521.  precond = 1 uses the Jacobi diagonal of
+ * -1/2 Lap (per radius) for M (the reference passes the ILU factors of
+ * S.Lap_std), 0 uses none.  use_guess = 0 starts from X = 0.
+ * Per-column iteration counts and relative residuals go to col_iters and
+ * col_relres (host, may be NULL).  Returns CYX_ERR_NOCONV (with X and info
+ * filled) when a column misses tol within max_iter or breaks down. */
+int cyx_sternheimer_solve(cyx_ham_t ham, int dtype, const void* B, void* X, int ncol, int64_t ld, const double* lam,
+                          const void* psi_v, int64_t ldv, int nv, int use_guess, double tol, int max_iter,
+                          int precond, cyx_stern_info* info, int32_t* col_iters, double* col_relres);
+
 /* ------------------------------------------------------------ Poisson */
 typedef struct cyx_aar_info {
   int32_t iterations; /* count-1 at exit, as src/aar.m:394-397 reports */

@@ -726,3 +726,77 @@ def scf(g: Grid, W, atoms, channels, wkpt, bcharge, rho0, Eself, Ecorr, nelec, n
     converged = not (count_scf == maxit_scf + 1 and err > SCF_tol)
     return dict(converged=converged, count_SCF=count_scf - 1, Etotal=Etot, Eatom=Etot / n_atom, lambda_f=lam_f,
                 EigVal=EigVal, occ=np.stack(occs, 1), history=history, psi=psis, rho=rho, phi=phi, veff=veff)
+
+
+def bicgstab(A, b, x0, tol, maxit, Minv=None):
+    """BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992) 631),
+    right-preconditioned, with the half-step exit of MATLAB's bicgstab
+    (src/phonon.m:1519 calls it).  Stops when ||b - A x|| <= tol ||b|| by the
+    recursive residual.  Returns (x, relres, iterations, flag): flag 0
+    converged, 1 iteration limit, 4 breakdown."""
+    M = (lambda z: z) if Minv is None else Minv
+    x = np.zeros_like(b) if x0 is None else np.array(x0, copy=True)
+    bn = np.linalg.norm(b)
+    if bn == 0:
+        return np.zeros_like(b), 0.0, 0, 0
+    r = b - A(x)
+    rt = r.copy()
+    if np.linalg.norm(r) <= tol * bn:
+        return x, np.linalg.norm(r) / bn, 0, 0
+    rho_prev = alpha = omega = 1.0
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    for it in range(1, maxit + 1):
+        rho = np.vdot(rt, r)
+        if rho == 0:
+            return x, np.linalg.norm(r) / bn, it, 4
+        if it == 1:
+            p = r.copy()
+        else:
+            beta = (rho / rho_prev) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+        ph = M(p)
+        v = A(ph)
+        rtv = np.vdot(rt, v)
+        if rtv == 0:
+            return x, np.linalg.norm(r) / bn, it, 4
+        alpha = rho / rtv
+        s = r - alpha * v
+        if np.linalg.norm(s) <= tol * bn:
+            return x + alpha * ph, np.linalg.norm(s) / bn, it, 0
+        sh = M(s)
+        t = A(sh)
+        tt = np.vdot(t, t)
+        if tt == 0:
+            return x + alpha * ph, np.linalg.norm(s) / bn, it, 4
+        omega = np.vdot(t, s) / tt
+        x = x + alpha * ph + omega * sh
+        r = s - omega * t
+        if np.linalg.norm(r) <= tol * bn:
+            return x, np.linalg.norm(r) / bn, it, 0
+        if omega == 0:
+            return x, np.linalg.norm(r) / bn, it, 4
+        rho_prev = rho
+    return x, np.linalg.norm(r) / bn, maxit, 1
+
+
+def jacobi_lap_half(g: Grid):
+    """Per-point inverse of the diagonal of -1/2 Lap (the Jacobi stand-in for
+    the ILU factors of S.Lap_std that src/phonon.m:1519 passes)."""
+    d = -0.5 * lap_matrix(g).diagonal()
+    return 1.0 / d
+
+
+def sternheimer_solve(ham: Hamiltonian, lam, psi_v, B, X0=None, tol=1e-8, maxit=3000, precond=True):
+    """Column by column bicgstab on (H - lam_n + psi_v psi_v^H W) x_n = b_n
+    (src/phonon.m:1516-1521).  Returns (X, relres[], iterations[], flags[])."""
+    dinv = jacobi_lap_half(ham.g) if precond else None
+    Minv = (lambda z: dinv * z) if precond else None
+    ncol = B.shape[1]
+    X = np.zeros_like(B)
+    rel, its, flags = np.zeros(ncol), np.zeros(ncol, dtype=int), np.zeros(ncol, dtype=int)
+    for j in range(ncol):
+        A = lambda z, j=j: sternheimer_apply(ham, [lam[j]], psi_v, z[:, None])[:, 0]
+        x0 = None if X0 is None else X0[:, j]
+        X[:, j], rel[j], its[j], flags[j] = bicgstab(A, B[:, j], x0, tol, maxit, Minv)
+    return X, rel, its, flags
