@@ -26,6 +26,9 @@ EXPORTS = (
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
     "cyx_sternheimer_apply", "cyx_sternheimer_solve",
+    "cyx_unet_create", "cyx_unet_destroy", "cyx_unet_num_params", "cyx_unet_set_params", "cyx_unet_get_params",
+    "cyx_unet_forward", "cyx_unet_gradient", "cyx_unet_train_step", "cyx_unet_input",
+    "cyx_poisson_set_preconditioner",
 )
 
 
@@ -107,6 +110,16 @@ def load():
         "cyx_mixer_create": (i32, [vp, i64, i32, dbl, dbl, i32, vp, C.POINTER(vp)]),
         "cyx_mixer_destroy": (i32, [vp]),
         "cyx_mixer_step": (i32, [vp, vp, vp, vp, i32]),
+        "cyx_unet_create": (i32, [vp, C.POINTER(GridDesc), C.POINTER(vp)]),
+        "cyx_unet_destroy": (i32, [vp]),
+        "cyx_unet_num_params": (i64, [vp]),
+        "cyx_unet_set_params": (i32, [vp, C.POINTER(C.c_float), i32]),
+        "cyx_unet_get_params": (i32, [vp, C.POINTER(C.c_float)]),
+        "cyx_unet_forward": (i32, [vp, vp, vp, i32]),
+        "cyx_unet_gradient": (i32, [vp, vp, vp, i32, C.POINTER(C.c_float), pd]),
+        "cyx_unet_train_step": (i32, [vp, vp, vp, i32, C.c_float, pd]),
+        "cyx_unet_input": (i32, [vp, vp, vp, vp, i32, vp]),
+        "cyx_poisson_set_preconditioner": (i32, [vp, vp]),
         "cyx_sternheimer_apply": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32]),
         "cyx_sternheimer_solve": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32, i32, dbl, i32, i32,
                                         C.POINTER(SternInfo), C.POINTER(C.c_int32), pd]),

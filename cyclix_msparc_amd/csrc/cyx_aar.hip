@@ -103,6 +103,16 @@ __device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x
   return acc;
 }
 
+// r1 = f - Lap x (the network's input on learned-preconditioner iterations)
+__global__ void __launch_bounds__(AB) resid_kernel(PoissonOp P, const double* __restrict__ f,
+                                                   const double* __restrict__ x, const double* __restrict__ T,
+                                                   double* __restrict__ r1, const AarState* S) {
+  if (S->done) return;
+  const Pt q = point_of(P.nr, P.nt);
+  if (!q.ok) return;
+  r1[q.n] = f[q.n] - lap_at(P, x, T, q.i, q.j, q.k, q.n);
+}
+
 __global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, int64_t N, double* part) {
   __shared__ double red[AB];
   double a = 0;
@@ -132,6 +142,8 @@ struct StepArgs {
   int anderson;         // 1 on Anderson steps
   int count, max_iter, xbuf;
   double tol, omega;
+  const double* r1buf;  // learned preconditioner: f - Lap x_prev, computed before the network
+  const float* u;       //   and the network's correction U(D^{-1} r1) (null: Jacobi only)
 };
 
 // MH: compile-time history size for the Anderson dot products (0: none)
@@ -149,9 +161,10 @@ __global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
   if (q.ok) {
     const int n = q.n;
     const double x = A.xp[n];
-    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
+    const double r1 = A.u ? A.r1buf[n] : A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
     nrm = r1 * r1;
-    const double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
+    double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
+    if (A.u) r += (double)A.u[n];
     A.res[n] = r;
     if (A.hcol >= 0) {
       A.DX[(int64_t)A.hcol * N + n] = x - A.xo[n];
@@ -282,9 +295,11 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
   const int nblk = (int)(pgrid.x * pgrid.y * pgrid.z);
   const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
   const int ndmax = 1 + m * (m + 1) / 2 + m;
-  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state
+  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
   const size_t vec = (size_t)N * sizeof(double);
-  const size_t need = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
+  cyx_unet_s* net = ps->net;
+  const size_t state_end = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
+  const size_t need = state_end + (net ? vec + (size_t)N * sizeof(float) + 256 : 0);
   if (ps->scratch_bytes < need) {
     if (ps->scratch) (void)hipFree(ps->scratch);
     ps->scratch = nullptr;
@@ -302,6 +317,8 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
   double* DF = DX + (int64_t)m * N;
   double* part = DF + (int64_t)m * N;
   AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * ndmax) + 255) & ~(uintptr_t)255);
+  double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
+  float* ubuf = net ? (float*)(r1buf + N) : nullptr;
   double* X[2] = {xalt, x};  // X[1] = caller's x (guess), X[0] = xalt
   if (!use_guess) hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, x, N, 1.0);
   CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
@@ -336,7 +353,9 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
   AarState hs{};
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
-  const int chunk = 16;
+  // iterations enqueued between polls of the device flag (the network's
+  // kernels do not test it, so the learned path polls more often)
+  const int chunk = net ? 4 : 16;
   const dim3 grid = pgrid;
   while (count <= max_iter) {
     const int cend = std::min(max_iter, count + chunk - 1);
@@ -360,7 +379,15 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
       A.xbuf = xb;
       A.tol = tol;
       A.omega = omega;
+      A.r1buf = r1buf;
+      A.u = ubuf;
       if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, A.xp, T, (const AarState*)S);
+      if (net) {
+        hipLaunchKernelGGL(resid_kernel, grid, dim3(TX, TY), 0, st, P, f, A.xp, (const double*)T, r1buf,
+                           (const AarState*)S);
+        int rc = unet_precondition(net, ps->coef, r1buf, ubuf);
+        if (rc != CYX_OK) return rc;
+      }
       if (!A.anderson) {
         hipLaunchKernelGGL(aar_step_kernel<0>, grid, dim3(TX, TY), 0, st, P, A);
         hipLaunchKernelGGL(aar_finalize_kernel<0>, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, 0, count,

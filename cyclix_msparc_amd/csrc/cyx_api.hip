@@ -847,6 +847,16 @@ int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* 
   return CYX_OK;
 }
 
+int cyx_poisson_set_preconditioner(cyx_poisson_t p, cyx_unet_t net) {
+  CYX_CHECK(p, "null argument");
+  if (net) {
+    CYX_CHECK(unet_matches(net, p->g), "the U-Net was built for another grid");
+    CYX_CHECK(unet_context(net) == p->ctx, "the U-Net and the Poisson handle must share a context");
+  }
+  p->net = net;
+  return CYX_OK;
+}
+
 int cyx_poisson_destroy(cyx_poisson_t p) {
   if (!p) return CYX_OK;
   (void)hipDeviceSynchronize();

@@ -121,6 +121,7 @@ struct cyx_ham_s {
 
 struct cyx_poisson_s {
   cyx_ctx_s* ctx = nullptr;
+  cyx_unet_s* net = nullptr;  // learned preconditioner (null: Jacobi only)
   cyx_grid_desc g{};
   double* coef = nullptr;   // [nr][RCOEF] Lap_std coefficients (no -1/2)
   double* diag = nullptr;   // [nr] diagonal of Lap_std per radius
@@ -147,6 +148,11 @@ int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const
 int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
 int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* scal /*dev ns*/);
 int hermitize(cyx_ham_s* h, int dtype, void* M, int ns);
+
+// learned preconditioner: u = U(D^{-1} r) (FP32), see cyx_unet.hip
+int unet_precondition(cyx_unet_s* net, const double* coef, const double* r, float* u);
+bool unet_matches(const cyx_unet_s* net, const cyx_grid_desc& g);
+cyx_ctx_s* unet_context(const cyx_unet_s* net);
 
 // Poisson kernels
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,

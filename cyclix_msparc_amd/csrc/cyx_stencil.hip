@@ -20,10 +20,17 @@
 
 namespace cyx {
 
-// workgroup size cap per element size: complex128 needs ~200 VGPRs per lane
+// workgroup size cap: 512 threads (8 waves).  Types up to 8 bytes fit in
+// 128 VGPRs, so two workgroups share a CU (4 waves per SIMD) and one's
+// barrier waits overlap the other's work; complex128 needs ~220 VGPRs and
+// runs one workgroup per CU.
 template <class T>
 constexpr int stencil_max_threads() {
-  return sizeof(T) >= 16 ? 512 : 768;
+  return 512;
+}
+template <class T>
+constexpr int stencil_min_blocks() {
+  return sizeof(T) >= 16 ? 1 : 2;
 }
 
 template <class T>
@@ -78,7 +85,8 @@ __device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
 }
 
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
-__global__ void __launch_bounds__(stencil_max_threads<T>()) stencil_step_kernel(const StepParams<T> P) {
+__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
+    stencil_step_kernel(const StepParams<T> P) {
   using R = typename traits<T>::real;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);

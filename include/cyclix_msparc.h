@@ -78,6 +78,7 @@ typedef struct cyx_ctx_s* cyx_ctx_t;
 typedef struct cyx_ham_s* cyx_ham_t;
 typedef struct cyx_poisson_s* cyx_poisson_t;
 typedef struct cyx_mixer_s* cyx_mixer_t;
+typedef struct cyx_unet_s* cyx_unet_t;
 
 /* ------------------------------------------------------------ context */
 const char* cyx_last_error(void);
@@ -241,6 +242,34 @@ int cyx_aar(cyx_poisson_t ps, const double* f, double* x, int use_guess, double 
  * rho, b, phi device FP64 length N. */
 int cyx_poisson_solve(cyx_poisson_t ps, const double* rho, const double* b, double* phi, int is_guess, double tol,
                       int max_iter, cyx_aar_info* info);
+
+/* ------------------------------------------------------------ learned Poisson preconditioner */
+/* A linear three-level 3-D convolutional U-Net U on the handle grid (nr, nt,
+ * nz divisible by 4): BF16 weights, BF16 matrix-core products, FP32
+ * accumulation.  With it, the AAR preconditioner of a Poisson handle becomes
+ * M r = D^{-1} r + U(D^{-1} r), D the Jacobi diagonal of Lap_std, in place of
+ * the reference's ILU factors (src/poissonSolve.m:36).  It has no counterpart
+ * in the reference.  Parameters: one FP32 vector (cyx_unet_num_params
+ * entries: the layer weights, then a per-radius output scale). */
+int cyx_unet_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_unet_t* out);
+int cyx_unet_destroy(cyx_unet_t net);
+int64_t cyx_unet_num_params(cyx_unet_t net);
+/* host FP32 parameters in/out; reset_optimizer zeroes the Adam state */
+int cyx_unet_set_params(cyx_unet_t net, const float* params, int reset_optimizer);
+int cyx_unet_get_params(cyx_unet_t net, float* params);
+/* u = U(x) for nb samples (device FP32, nb x N each) */
+int cyx_unet_forward(cyx_unet_t net, const float* x, float* u, int nb);
+/* loss = (1/nb) sum_b ||U(x_b) - t_b||^2 / ||t_b||^2 and its gradient with
+ * respect to the parameters (host FP32, cyx_unet_num_params entries) */
+int cyx_unet_gradient(cyx_unet_t net, const float* x, const float* target, int nb, float* grad, double* loss);
+/* one Adam step (beta 0.9 / 0.999) on that loss; loss before the step */
+int cyx_unet_train_step(cyx_unet_t net, const float* x, const float* target, int nb, float lr, double* loss);
+/* x_b = scale_b * r_b / D (the network's input for residuals r, FP64 device,
+ * nb x N; scale: device FP32 nb values, or NULL for 1) */
+int cyx_unet_input(cyx_unet_t net, const cyx_poisson_t ps, const double* r, const float* scale, int nb, float* x);
+/* use net (NULL: Jacobi only) in every later cyx_aar / cyx_poisson_solve on ps;
+ * net must live on the same context and grid */
+int cyx_poisson_set_preconditioner(cyx_poisson_t ps, cyx_unet_t net);
 
 #ifdef __cplusplus
 }
