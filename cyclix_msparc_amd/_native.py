@@ -28,7 +28,8 @@ EXPORTS = (
     "cyx_sternheimer_apply", "cyx_sternheimer_solve",
     "cyx_unet_create", "cyx_unet_destroy", "cyx_unet_num_params", "cyx_unet_set_params", "cyx_unet_get_params",
     "cyx_unet_forward", "cyx_unet_gradient", "cyx_unet_train_step", "cyx_unet_input",
-    "cyx_poisson_set_preconditioner",
+    "cyx_poisson_set_preconditioner", "cyx_poisson_apply",
+    "cyx_filter_plan_create", "cyx_filter_plan_run", "cyx_filter_plan_tuning", "cyx_filter_plan_destroy",
 )
 
 
@@ -58,7 +59,8 @@ class Tuning(C.Structure):
 
 
 class AarInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double)]
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double),
+                ("precond", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SternInfo(C.Structure):
@@ -120,6 +122,13 @@ def load():
         "cyx_unet_train_step": (i32, [vp, vp, vp, i32, C.c_float, pd]),
         "cyx_unet_input": (i32, [vp, vp, vp, vp, i32, vp]),
         "cyx_poisson_set_preconditioner": (i32, [vp, vp]),
+        "cyx_poisson_apply": (i32, [vp, vp, vp]),
+        "cyx_filter_plan_create": (i32, [vp, i32, C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(vp),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int64), i32, i32, pd, pd, pd,
+                                         C.POINTER(vp)]),
+        "cyx_filter_plan_run": (i32, [vp, pd, pd, pd, C.POINTER(C.c_int32)]),
+        "cyx_filter_plan_tuning": (i32, [vp, C.POINTER(Tuning)]),
+        "cyx_filter_plan_destroy": (i32, [vp]),
         "cyx_sternheimer_apply": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32]),
         "cyx_sternheimer_solve": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32, i32, dbl, i32, i32,
                                         C.POINTER(SternInfo), C.POINTER(C.c_int32), pd]),

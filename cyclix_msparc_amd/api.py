@@ -166,9 +166,10 @@ class Hamiltonian:
             v = np.ascontiguousarray(np.asarray(veff, dtype=float))
             nat.check(self.lib.cyx_ham_set_veff(self.h, v.ctypes.data_as(C.c_void_p), 0))
 
-    def set_tuning(self, tr=0, zb=0, nontemporal=False, no_xcd_remap=False):
+    def set_tuning(self, tr=0, zb=0, nontemporal=False, no_xcd_remap=False, tile_major=False):
         """Launch shape of the fused stencil (results do not depend on it)."""
-        t = nat.Tuning(int(tr), int(zb), (1 if nontemporal else 0) | (2 if no_xcd_remap else 0), 0)
+        t = nat.Tuning(int(tr), int(zb), (1 if nontemporal else 0) | (2 if no_xcd_remap else 0)
+                       | (4 if tile_major else 0), 0)
         nat.check(self.lib.cyx_ham_set_tuning(self.h, C.byref(t)))
 
     def new_block(self, ns, dtype_code):
@@ -290,6 +291,72 @@ class Hamiltonian:
         return lo.value, hi.value
 
 
+class FilterPlan:
+    """Batched Chebyshev filter over several channels: one launch per
+    recurrence step for all channels, the pass captured in a HIP graph
+    (cyx_filter_plan_*).  blocks: list of (ncol, N) device tensors (one per
+    channel, same dtype); each is filtered in place (the result is copied
+    back into it when the recurrence ends in the work block)."""
+
+    def __init__(self, dev: Device, hams, blocks, m, autotune=False, bounds=None):
+        self.dev = dev
+        self.lib = dev.lib
+        self.hams = list(hams)
+        self.blocks = list(blocks)
+        if len(self.hams) != len(self.blocks) or not self.hams:
+            raise ValueError("one block per Hamiltonian")
+        dt = self.blocks[0].dtype
+        for h, X in zip(self.hams, self.blocks):
+            _check_block(X, h.N)
+            if X.dtype != dt:
+                raise ValueError("all blocks must share a dtype")
+        self.dtype_code = _dtype_code(self.blocks[0])
+        self.work = [torch.empty_like(X) for X in self.blocks]
+        n = len(self.hams)
+        hv = (C.c_void_p * n)(*[h.h.value for h in self.hams])
+        av = (C.c_void_p * n)(*[X.data_ptr() for X in self.blocks])
+        bv = (C.c_void_p * n)(*[W.data_ptr() for W in self.work])
+        nc = (C.c_int32 * n)(*[X.shape[0] for X in self.blocks])
+        ld = (C.c_int64 * n)(*[h.N for h in self.hams])
+        a = b = a0 = None
+        if autotune:
+            a, b, a0 = (np.ascontiguousarray(np.asarray(v, dtype=float)) for v in bounds)
+        h = C.c_void_p()
+        nat.check(self.lib.cyx_filter_plan_create(self.dev.h, n, hv, self.dtype_code, av, bv, nc, ld, int(m),
+                                                  1 if autotune else 0, nat.dptr(a) if a is not None else None,
+                                                  nat.dptr(b) if b is not None else None,
+                                                  nat.dptr(a0) if a0 is not None else None, C.byref(h)))
+        self.h = h
+        self.m = int(m)
+
+    def tuning(self):
+        t = nat.Tuning()
+        nat.check(self.lib.cyx_filter_plan_tuning(self.h, C.byref(t)))
+        return {"tr": t.tr, "flags": t.flags}
+
+    def run(self, a, b, a0):
+        """Filter every block in place with per-channel bounds."""
+        a, b, a0 = (np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=float), (len(self.hams),)))
+                    for v in (a, b, a0))
+        res = C.c_int32()
+        nat.check(self.lib.cyx_filter_plan_run(self.h, nat.dptr(a), nat.dptr(b), nat.dptr(a0), C.byref(res)))
+        if res.value:
+            for X, W in zip(self.blocks, self.work):
+                X.copy_(W)
+        return self.blocks
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cyx_filter_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def lda_pw(dev: Device, rho, phi=None, rhotol=1e-14):
     """(vxc, veff): LDA-PW potential (``src/exchangeCorrelationPotential.m:82-106``)
     and, with phi, Veff = phi + Vxc (``src/scf.m:39``); device FP64 vectors."""
@@ -366,6 +433,13 @@ class Poisson:
             self.close()
         except Exception:
             pass
+
+    def apply(self, x):
+        """y = Lap_std x."""
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        nat.check(self.lib.cyx_poisson_apply(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        return y
 
     def aar(self, f, x_guess=None, tol=1e-8, max_iter=1000, omega=0.6, beta=0.6, m=7, p=6):
         """``src/aar.m:33-72``; returns (x, info).  Raises on non-convergence

@@ -103,7 +103,13 @@ __device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x
   return acc;
 }
 
-// r1 = f - Lap x (the network's input on learned-preconditioner iterations)
+// ---- learned correction with exact line search (Anderson iterations):
+//   r1 = f - Lap x,  d = r1 ./ diag + U(r1 ./ diag),  w = Lap d,
+//   delta = <w, r1> / <w, w>,  x += delta d
+// delta minimises ||f - Lap (x + delta d)||, so the step never raises the
+// residual norm whatever the network returns.
+
+// r1 = f - Lap x
 __global__ void __launch_bounds__(AB) resid_kernel(PoissonOp P, const double* __restrict__ f,
                                                    const double* __restrict__ x, const double* __restrict__ T,
                                                    double* __restrict__ r1, const AarState* S) {
@@ -111,6 +117,50 @@ __global__ void __launch_bounds__(AB) resid_kernel(PoissonOp P, const double* __
   const Pt q = point_of(P.nr, P.nt);
   if (!q.ok) return;
   r1[q.n] = f[q.n] - lap_at(P, x, T, q.i, q.j, q.k, q.n);
+}
+
+__global__ void __launch_bounds__(AB) ls_dir_kernel(PoissonOp P, const double* __restrict__ r1,
+                                                    const float* __restrict__ u, double* __restrict__ d,
+                                                    const AarState* S) {
+  if (S->done) return;
+  const Pt q = point_of(P.nr, P.nt);
+  if (!q.ok) return;
+  d[q.n] = r1[q.n] / P.coef[q.i * RCOEF + 2 * FDN + 1] + (double)u[q.n];
+}
+
+__global__ void __launch_bounds__(AB) ls_apply_kernel(PoissonOp P, const double* __restrict__ d,
+                                                      const double* __restrict__ T, const double* __restrict__ r1,
+                                                      double* __restrict__ part, const AarState* S) {
+  if (S->done) return;
+  __shared__ double red[AB];
+  const Pt q = point_of(P.nr, P.nt);
+  double a = 0, b = 0;
+  if (q.ok) {
+    const double w = lap_at(P, d, T, q.i, q.j, q.k, q.n);
+    a = w * r1[q.n];
+    b = w * w;
+  }
+  const double sa = block_sum(a, red);
+  const double sb = block_sum(b, red);
+  if (tlin() == 0) {
+    part[(int64_t)blin() * 2] = sa;
+    part[(int64_t)blin() * 2 + 1] = sb;
+  }
+}
+
+__global__ void __launch_bounds__(AB) ls_finalize_kernel(const double* part, int nblk, AarState* S) {
+  if (S->done) return;
+  __shared__ double tot[2];
+  reduce_partials(part, nblk, 2, tot);
+  if (threadIdx.x == 0) S->gamma[15] = tot[1] > 0 ? tot[0] / tot[1] : 0.0;
+}
+
+__global__ void __launch_bounds__(AB) ls_axpy_kernel(PoissonOp P, double* __restrict__ x,
+                                                     const double* __restrict__ d, const AarState* S) {
+  if (S->done) return;
+  const Pt q = point_of(P.nr, P.nt);
+  if (!q.ok) return;
+  x[q.n] += S->gamma[15] * d[q.n];
 }
 
 __global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, int64_t N, double* part) {
@@ -142,8 +192,6 @@ struct StepArgs {
   int anderson;         // 1 on Anderson steps
   int count, max_iter, xbuf;
   double tol, omega;
-  const double* r1buf;  // learned preconditioner: f - Lap x_prev, computed before the network
-  const float* u;       //   and the network's correction U(D^{-1} r1) (null: Jacobi only)
 };
 
 // MH: compile-time history size for the Anderson dot products (0: none)
@@ -161,10 +209,9 @@ __global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
   if (q.ok) {
     const int n = q.n;
     const double x = A.xp[n];
-    const double r1 = A.u ? A.r1buf[n] : A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
+    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
     nrm = r1 * r1;
-    double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
-    if (A.u) r += (double)A.u[n];
+    const double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
     A.res[n] = r;
     if (A.hcol >= 0) {
       A.DX[(int64_t)A.hcol * N + n] = x - A.xo[n];
@@ -285,47 +332,7 @@ static void launch_anderson(dim3 grid, int nblk, hipStream_t st, const PoissonOp
                      xnew, (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
 }
 
-int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
-                double omega, double beta, int m, int p, cyx_aar_info* info) {
-  CYX_CHECK(m >= 1 && m <= 8, "AAR history m must be in [1, 8]");
-  CYX_CHECK(p >= 1 && max_iter >= 1 && tol > 0, "invalid AAR parameters");
-  const int64_t N = ps->N();
-  hipStream_t st = ps->ctx->stream;
-  const dim3 pgrid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
-  const int nblk = (int)(pgrid.x * pgrid.y * pgrid.z);
-  const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
-  const int ndmax = 1 + m * (m + 1) / 2 + m;
-  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
-  const size_t vec = (size_t)N * sizeof(double);
-  cyx_unet_s* net = ps->net;
-  const size_t state_end = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
-  const size_t need = state_end + (net ? vec + (size_t)N * sizeof(float) + 256 : 0);
-  if (ps->scratch_bytes < need) {
-    if (ps->scratch) (void)hipFree(ps->scratch);
-    ps->scratch = nullptr;
-    ps->scratch_bytes = 0;
-    if (hipMalloc((void**)&ps->scratch, need) != hipSuccess) {
-      set_error("poisson scratch allocation of %zu bytes failed", need);
-      return CYX_ERR_NOMEM;
-    }
-    ps->scratch_bytes = need;
-  }
-  double* xalt = ps->scratch;
-  double* R[2] = {xalt + N, xalt + 2 * N};
-  double* T = xalt + 3 * N;
-  double* DX = xalt + 4 * N;
-  double* DF = DX + (int64_t)m * N;
-  double* part = DF + (int64_t)m * N;
-  AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * ndmax) + 255) & ~(uintptr_t)255);
-  double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
-  float* ubuf = net ? (float*)(r1buf + N) : nullptr;
-  double* X[2] = {xalt, x};  // X[1] = caller's x (guess), X[0] = xalt
-  if (!use_guess) hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, x, N, 1.0);
-  CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
-  CYX_HIP(hipMemsetAsync(T, 0, vec, st));
-  CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
-  hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part);
-  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk_norm, S);
+static PoissonOp make_op(const cyx_poisson_s* ps) {
   PoissonOp P{};
   P.nr = ps->g.nr;
   P.nt = ps->g.nt;
@@ -350,12 +357,81 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
       P.wz1[k] = w1[k] / ps->g.dz;
     }
   }
+  return P;
+}
+
+// y = Lap_std x
+__global__ void __launch_bounds__(AB) apply_kernel(PoissonOp P, const double* __restrict__ x,
+                                                   const double* __restrict__ T, double* __restrict__ y) {
+  const Pt q = point_of(P.nr, P.nt);
+  if (!q.ok) return;
+  y[q.n] = lap_at(P, x, T, q.i, q.j, q.k, q.n);
+}
+
+int poisson_apply(cyx_poisson_s* ps, const double* x, double* y) {
+  const int64_t N = ps->N();
+  hipStream_t st = ps->ctx->stream;
+  const PoissonOp P = make_op(ps);
+  const dim3 grid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
+  CYX_CHECK(ensure(ps->tbuf, (size_t)N * sizeof(double) + sizeof(AarState)) == CYX_OK, "allocation failed");
+  double* T = (double*)ps->tbuf.p;
+  AarState* S = (AarState*)(T + N);
+  CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
+  if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, x, T, (const AarState*)S);
+  hipLaunchKernelGGL(apply_kernel, grid, dim3(TX, TY), 0, st, P, x, (const double*)T, y);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
+                    double omega, double beta, int m, int p, cyx_aar_info* info, cyx_unet_s* net, bool* diverged) {
+  *diverged = false;
+  CYX_CHECK(m >= 1 && m <= 8, "AAR history m must be in [1, 8]");
+  CYX_CHECK(p >= 1 && max_iter >= 1 && tol > 0, "invalid AAR parameters");
+  const int64_t N = ps->N();
+  hipStream_t st = ps->ctx->stream;
+  const dim3 pgrid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
+  const int nblk = (int)(pgrid.x * pgrid.y * pgrid.z);
+  const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
+  const int ndmax = 1 + m * (m + 1) / 2 + m;
+  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
+  const size_t vec = (size_t)N * sizeof(double);
+  const size_t state_end = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
+  const size_t need = state_end + (net ? 2 * vec + (size_t)N * sizeof(float) + 256 : 0);
+  if (ps->scratch_bytes < need) {
+    if (ps->scratch) (void)hipFree(ps->scratch);
+    ps->scratch = nullptr;
+    ps->scratch_bytes = 0;
+    if (hipMalloc((void**)&ps->scratch, need) != hipSuccess) {
+      set_error("poisson scratch allocation of %zu bytes failed", need);
+      return CYX_ERR_NOMEM;
+    }
+    ps->scratch_bytes = need;
+  }
+  double* xalt = ps->scratch;
+  double* R[2] = {xalt + N, xalt + 2 * N};
+  double* T = xalt + 3 * N;
+  double* DX = xalt + 4 * N;
+  double* DF = DX + (int64_t)m * N;
+  double* part = DF + (int64_t)m * N;
+  AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * ndmax) + 255) & ~(uintptr_t)255);
+  double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
+  double* dbuf = net ? r1buf + N : nullptr;
+  float* ubuf = net ? (float*)(dbuf + N) : nullptr;
+  double* X[2] = {xalt, x};  // X[1] = caller's x (guess), X[0] = xalt
+  if (!use_guess) hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, x, N, 1.0);
+  CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
+  CYX_HIP(hipMemsetAsync(T, 0, vec, st));
+  CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
+  hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part);
+  hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk_norm, S);
+  const PoissonOp P = make_op(ps);
   AarState hs{};
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
   // iterations enqueued between polls of the device flag (the network's
   // kernels do not test it, so the learned path polls more often)
-  const int chunk = net ? 4 : 16;
+  const int chunk = net ? 6 : 16;
   const dim3 grid = pgrid;
   while (count <= max_iter) {
     const int cend = std::min(max_iter, count + chunk - 1);
@@ -379,15 +455,7 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
       A.xbuf = xb;
       A.tol = tol;
       A.omega = omega;
-      A.r1buf = r1buf;
-      A.u = ubuf;
       if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, A.xp, T, (const AarState*)S);
-      if (net) {
-        hipLaunchKernelGGL(resid_kernel, grid, dim3(TX, TY), 0, st, P, f, A.xp, (const double*)T, r1buf,
-                           (const AarState*)S);
-        int rc = unet_precondition(net, ps->coef, r1buf, ubuf);
-        if (rc != CYX_OK) return rc;
-      }
       if (!A.anderson) {
         hipLaunchKernelGGL(aar_step_kernel<0>, grid, dim3(TX, TY), 0, st, P, A);
         hipLaunchKernelGGL(aar_finalize_kernel<0>, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, 0, count,
@@ -403,12 +471,34 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
           case 7: launch_anderson<7>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
           default: launch_anderson<8>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
         }
+        if (net) {
+          // learned correction of the Anderson iterate, with exact line search
+          double* xn = X[xb ^ 1];
+          const AarState* cS = S;
+          if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)xn, T, cS);
+          hipLaunchKernelGGL(resid_kernel, grid, dim3(TX, TY), 0, st, P, f, (const double*)xn, (const double*)T,
+                             r1buf, cS);
+          int rc = unet_precondition(net, ps->coef, r1buf, ubuf);
+          if (rc != CYX_OK) return rc;
+          hipLaunchKernelGGL(ls_dir_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)r1buf, (const float*)ubuf,
+                             dbuf, cS);
+          if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)dbuf, T, cS);
+          hipLaunchKernelGGL(ls_apply_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)dbuf, (const double*)T,
+                             (const double*)r1buf, part, cS);
+          hipLaunchKernelGGL(ls_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, S);
+          hipLaunchKernelGGL(ls_axpy_kernel, grid, dim3(TX, TY), 0, st, P, xn, (const double*)dbuf, cS);
+        }
       }
     }
     CYX_HIP(hipGetLastError());
     CYX_HIP(hipMemcpyAsync(&hs, S, sizeof(AarState), hipMemcpyDeviceToHost, st));
     CYX_HIP(hipStreamSynchronize(st));
     if (hs.done) break;
+    // a learned preconditioner that makes the iteration blow up is abandoned
+    if (net && !(hs.relres <= 1e3)) {
+      *diverged = true;
+      return CYX_ERR_NOCONV;
+    }
   }
   if (hs.final_buf == 0) CYX_HIP(hipMemcpyAsync(x, xalt, vec, hipMemcpyDeviceToDevice, st));
   CYX_HIP(hipStreamSynchronize(st));
@@ -416,8 +506,28 @@ int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, do
     info->iterations = hs.iters;
     info->relres = hs.relres;
     info->converged = hs.done == 1;
+    info->precond = net ? 1 : 0;
   }
   return hs.done == 1 ? CYX_OK : CYX_ERR_NOCONV;
+}
+
+int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
+                double omega, double beta, int m, int p, cyx_aar_info* info) {
+  bool diverged = false;
+  cyx_unet_s* net = ps->net;
+  if (!net) return run_impl(ps, f, x, use_guess, tol, max_iter, omega, beta, m, p, info, nullptr, &diverged);
+  // keep the guess: a diverging learned solve is restarted with Jacobi
+  const int64_t N = ps->N();
+  hipStream_t st = ps->ctx->stream;
+  CYX_CHECK(ensure(ps->tbuf, 2 * (size_t)N * sizeof(double) + 1024) == CYX_OK, "allocation failed");
+  double* gbak = (double*)ps->tbuf.p + N + 128;
+  if (use_guess) CYX_HIP(hipMemcpyAsync(gbak, x, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int rc = run_impl(ps, f, x, use_guess, tol, max_iter, omega, beta, m, p, info, net, &diverged);
+  if (!diverged) return rc;
+  if (use_guess) CYX_HIP(hipMemcpyAsync(x, gbak, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  rc = run_impl(ps, f, x, use_guess, tol, max_iter, omega, beta, m, p, info, nullptr, &diverged);
+  if (info) info->precond = 2;
+  return rc;
 }
 
 }  // namespace cyx

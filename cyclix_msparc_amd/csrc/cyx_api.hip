@@ -131,6 +131,8 @@ template <class T>
 struct HostKb {
   using R = typename traits<T>::real;
   std::vector<int> g_ptr, g_node, s_ptr, s_il, il_proj;
+  std::vector<uint32_t> s_pk;
+  int max_count = 0;
   std::vector<T> g_coef, il_phase;
   std::vector<R> gamma, s_chi;
 };
@@ -199,6 +201,13 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
       K.s_chi[k] = (R)e.chi;
       K.s_il[k] = e.il;
     }
+  // one word per node for the stencil: (first entry << 6) | count
+  K.s_pk.assign(N, 0u);
+  for (int64_t n = 0; n < N; ++n) {
+    const int c = K.s_ptr[n + 1] - K.s_ptr[n];
+    K.max_count = std::max(K.max_count, c);
+    K.s_pk[n] = ((uint32_t)K.s_ptr[n] << 6) | (uint32_t)std::min(c, 63);
+  }
 }
 
 template <class V>
@@ -216,6 +225,8 @@ template <class T>
 static int upload_kb(KbTables<T>& D, const cyx_projector_desc* d, const double* W, const double kpt[3], int64_t N) {
   HostKb<T> K;
   build_kb<T>(d, W, kpt, N, K);
+  CYX_CHECK(K.max_count <= 63, "a grid node lies in %d projector balls (at most 63 supported)", K.max_count);
+  CYX_CHECK(K.s_chi.size() < (size_t)1 << 26, "too many projector entries (%zu)", K.s_chi.size());
   D.nproj = (int)K.gamma.size();
   D.nimgl = (int)K.il_proj.size();
   int rc = CYX_OK;
@@ -223,7 +234,7 @@ static int upload_kb(KbTables<T>& D, const cyx_projector_desc* d, const double* 
   if ((rc = upload_vec(D.g_node, K.g_node))) return rc;
   if ((rc = upload_vec(D.g_coef, K.g_coef))) return rc;
   if ((rc = upload_vec(D.gamma, K.gamma))) return rc;
-  if ((rc = upload_vec(D.s_ptr, K.s_ptr))) return rc;
+  if ((rc = upload_vec(D.s_pk, K.s_pk))) return rc;
   if ((rc = upload_vec(D.s_chi, K.s_chi))) return rc;
   if ((rc = upload_vec(D.s_il, K.s_il))) return rc;
   if ((rc = upload_vec(D.il_phase, K.il_phase))) return rc;
@@ -237,7 +248,7 @@ static void free_kb(KbTables<T>& D) {
   (void)hipFree(D.g_node);
   (void)hipFree(D.g_coef);
   (void)hipFree(D.gamma);
-  (void)hipFree(D.s_ptr);
+  (void)hipFree(D.s_pk);
   (void)hipFree(D.s_chi);
   (void)hipFree(D.s_il);
   (void)hipFree(D.il_phase);
@@ -622,6 +633,7 @@ int cyx_ham_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, const double kpt[3]
   CYX_HIP(hipSetDevice(ctx->device));
   cyx_ham_s* h = new cyx_ham_s();
   h->ctx = ctx;
+  h->device = ctx->device;
   h->g = *grid;
   for (int d = 0; d < 3; ++d) h->kpt[d] = kpt[d];
   h->bloch = kpt[1] != 0.0 || kpt[2] != 0.0;
@@ -700,7 +712,7 @@ int cyx_ham_set_veff(cyx_ham_t h, const double* veff, int on_device) {
 
 int cyx_ham_destroy(cyx_ham_t h) {
   if (!h) return CYX_OK;
-  (void)hipSetDevice(h->ctx->device);
+  (void)hipSetDevice(h->device);  // not through h->ctx: the context may already be gone
   (void)hipDeviceSynchronize();
   (void)hipFree(h->W);
   (void)hipFree(h->Wm);
@@ -725,7 +737,7 @@ const double* cyx_ham_weights(cyx_ham_t h) { return h ? h->W : nullptr; }
 
 int cyx_ham_set_tuning(cyx_ham_t h, const cyx_tuning* t) {
   CYX_CHECK(h && t, "null argument");
-  CYX_CHECK(t->tr >= 0 && t->zb >= 0 && t->flags >= 0 && t->flags < 4, "bad tuning values");
+  CYX_CHECK(t->tr >= 0 && t->zb >= 0 && t->flags >= 0 && t->flags < 8, "bad tuning values");
   h->tune_tr = t->tr;
   h->tune_zb = t->zb;
   h->tune_flags = t->flags;
@@ -847,6 +859,11 @@ int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* 
   return CYX_OK;
 }
 
+int cyx_poisson_apply(cyx_poisson_t p, const double* x, double* y) {
+  CYX_CHECK(p && x && y && x != y, "null or aliased argument");
+  return poisson_apply(p, x, y);
+}
+
 int cyx_poisson_set_preconditioner(cyx_poisson_t p, cyx_unet_t net) {
   CYX_CHECK(p, "null argument");
   if (net) {
@@ -862,6 +879,7 @@ int cyx_poisson_destroy(cyx_poisson_t p) {
   (void)hipDeviceSynchronize();
   (void)hipFree(p->coef);
   (void)hipFree(p->scratch);
+  (void)hipFree(p->tbuf.p);
   delete p;
   return CYX_OK;
 }

@@ -36,7 +36,7 @@ struct KbTables {
   R* gamma = nullptr;     // [nproj]
   // scatter, CSR by node: HY[n] += sum_e chi[e] * D[il[e]], where for one
   // column D[il] = e^{-i k.d_img} * C[proj(il)] over (image, projector) pairs
-  int* s_ptr = nullptr;   // [N+1]
+  uint32_t* s_pk = nullptr;  // [N] (first entry << 6) | entry count (count <= 63)
   R* s_chi = nullptr;     // [nent]
   int* s_il = nullptr;    // [nent] (image, projector) pair index
   int nimgl = 0;
@@ -94,6 +94,7 @@ struct PhaseTimer {
 
 struct cyx_ham_s {
   cyx_ctx_s* ctx = nullptr;
+  int device = 0;
   cyx_grid_desc g{};
   double kpt[3] = {0, 0, 0};
   bool bloch = false;     // any non-zero Bloch phase (requires complex dtypes)
@@ -127,6 +128,7 @@ struct cyx_poisson_s {
   double* diag = nullptr;   // [nr] diagonal of Lap_std per radius
   double* scratch = nullptr;
   size_t scratch_bytes = 0;
+  cyx::DeviceBuf tbuf;      // theta-derivative buffer of cyx_poisson_apply
   int64_t N() const { return (int64_t)g.nr * g.nt * g.nz; }
 };
 
@@ -158,6 +160,7 @@ cyx_ctx_s* unet_context(const cyx_unet_s* net);
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
                 double omega, double beta, int m, int p, cyx_aar_info* info);
 int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f);
+int poisson_apply(cyx_poisson_s* ps, const double* x, double* y);
 
 // SCF glue
 int density_accumulate(cyx_ham_s* h, int dtype, const void* psi, int ns, const double* occ_dev, double c, double* rho);

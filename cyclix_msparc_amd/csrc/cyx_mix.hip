@@ -209,7 +209,8 @@ int cyx_mixer_create(cyx_ctx_t ctx, int64_t N, int history, double beta, double 
 
 int cyx_mixer_destroy(cyx_mixer_t M) {
   if (!M) return CYX_OK;
-  (void)hipStreamSynchronize(M->ctx->stream);
+  // the context may already be gone: synchronise the device, not its stream
+  (void)hipDeviceSynchronize();
   (void)hipFree(M->buf);
   delete M;
   return CYX_OK;

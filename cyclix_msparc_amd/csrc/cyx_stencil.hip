@@ -45,6 +45,7 @@ struct StepParams {
   int ntiles_r, nblocks_z;
   int xcd_remap;
   int nontemporal;
+  int tile_major, ncol;
   R alpha, beta, shift;
   const R* rcoef;
   const R* veff;
@@ -53,7 +54,7 @@ struct StepParams {
   R xin, dr;
   T ph_tl, ph_tr, ph_zl, ph_zr;
   int trh;  // LDS row stride (elements): tr + halo + bank padding
-  const int* s_ptr;
+  const uint32_t* s_pk;
   const R* s_chi;
   const int* s_il;
   const T* il_phase;
@@ -84,25 +85,35 @@ __device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
   __builtin_nontemporal_store(v.y, q + 1);
 }
 
+// The body of one workgroup: block `bid0` of the `nblk` blocks of one
+// channel's launch (the batched kernel runs several channels in one grid).
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
-__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
-    stencil_step_kernel(const StepParams<T> P) {
+__device__ __forceinline__ void stencil_body(const StepParams<T>& P, int bid0, int nblk, T* lds) {
   using R = typename traits<T>::real;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* lds = reinterpret_cast<T*>(smem_raw);
-
   const int TRH = P.trh;
   const int plane_sz = TRH * (P.nt + 2 * FDN);
   T* Dsh = lds + 2 * plane_sz;  // per-column KB table D[il]
 
-  int bid = blockIdx.x;
+  int bid = bid0;
   if (P.xcd_remap) {
-    const int per = gridDim.x >> 3;
+    const int per = nblk >> 3;
     bid = (bid & 7) * per + (bid >> 3);
   }
-  const int tile = bid % P.ntiles_r;
-  const int zblk = (bid / P.ntiles_r) % P.nblocks_z;
-  const int s = bid / (P.ntiles_r * P.nblocks_z);
+  // block order: radial tile fastest, or (tile-major) the column fastest so
+  // that the blocks resident on one XCD share a tile's Veff, coefficients and
+  // projector tables in its L2
+  int tile, zblk, s;
+  if (P.tile_major) {
+    const int per_tile = P.nblocks_z * P.ncol;
+    tile = bid / per_tile;
+    const int rem = bid - tile * per_tile;
+    zblk = rem % P.nblocks_z;
+    s = rem / P.nblocks_z;
+  } else {
+    tile = bid % P.ntiles_r;
+    zblk = (bid / P.ntiles_r) % P.nblocks_z;
+    s = bid / (P.ntiles_r * P.nblocks_z);
+  }
 
   const int tid = threadIdx.x;
   const int i = tid % P.tr;
@@ -172,9 +183,7 @@ __global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T
   // pointers of planes (kk, kk+1, kk+2) and echi/eil hold up to KB_SLOTS
   // entries of plane kk.  The step consumes plane kk's entries, issues the
   // entry loads of kk+1 and the row-pointer loads of kk+3.
-  int sp_hold0 = 0, sp_hold1 = 0;
-  int sp_cur0 = 0, sp_cur1 = 0;
-  int sp_nxt0 = 0, sp_nxt1 = 0;
+  uint32_t sp_hold = 0, sp_cur = 0, sp_nxt = 0;  // packed (first << 6) | count
   R echi[KB_SLOTS];
   int eil[KB_SLOTS];
 #pragma unroll
@@ -219,34 +228,28 @@ __global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T
         if constexpr (HAS_KB) {
           if (inner && rvalid) {
             T kb = zero<T>();
-            const int cnt = sp_hold1 - sp_hold0;
+            const int h0 = (int)(sp_hold >> 6), cnt = (int)(sp_hold & 63u);
 #pragma unroll
             for (int e = 0; e < KB_SLOTS; ++e)
               if (e < cnt) kb = rfma(echi[e], Dsh[eil[e]], kb);
-            for (int e = sp_hold0 + KB_SLOTS; e < sp_hold1; ++e) kb = rfma(P.s_chi[e], Dsh[P.s_il[e]], kb);
+            for (int e = h0 + KB_SLOTS; e < h0 + cnt; ++e) kb = rfma(P.s_chi[e], Dsh[P.s_il[e]], kb);
             q[u % QN] = add(q[u % QN], kb);
           }
           // entries of plane kk+1 (row pointers loaded one step ago)
-          sp_hold0 = sp_cur0;
-          sp_hold1 = sp_cur1;
+          sp_hold = sp_cur;
           {
-            const int cnt = sp_hold1 - sp_hold0;
+            const int h0 = (int)(sp_hold >> 6), cnt = (int)(sp_hold & 63u);
 #pragma unroll
             for (int e = 0; e < KB_SLOTS; ++e)
               if (e < cnt) {
-                echi[e] = P.s_chi[sp_hold0 + e];
-                eil[e] = P.s_il[sp_hold0 + e];
+                echi[e] = P.s_chi[h0 + e];
+                eil[e] = P.s_il[h0 + e];
               }
           }
-          // row pointers: (cur, nxt) become planes (kk+2, kk+3)
-          sp_cur0 = sp_nxt0;
-          sp_cur1 = sp_nxt1;
-          sp_nxt0 = sp_nxt1 = 0;
-          if (rvalid && kk + 3 >= z0 && kk + 3 < z1) {
-            const int64_t n3 = (int64_t)(kk + 3) * nrt + jr;
-            sp_nxt0 = P.s_ptr[n3];
-            sp_nxt1 = P.s_ptr[n3 + 1];
-          }
+          // node words: (cur, nxt) become planes (kk+2, kk+3)
+          sp_cur = sp_nxt;
+          sp_nxt = 0;
+          if (rvalid && kk + 3 >= z0 && kk + 3 < z1) sp_nxt = P.s_pk[(int64_t)(kk + 3) * nrt + jr];
         }
         __syncthreads();
         // ---- prefetch: next plane, and X_prev of the plane emitted next step
@@ -324,6 +327,32 @@ __global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T
   }
 }
 
+template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
+    stencil_step_kernel(const StepParams<T> P) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  stencil_body<T, HELICAL, HAS_PREV, HAS_KB>(P, blockIdx.x, gridDim.x, reinterpret_cast<T*>(smem_raw));
+}
+
+// Several channels (Bloch phases, grids, column counts) in one launch: entry
+// c owns blocks [E[c].blk0, E[c+1].blk0).  All entries share nt and the r
+// tile (one workgroup shape) and the template variant.
+template <class T>
+struct BatchEntry {
+  StepParams<T> P;
+  int blk0, nblk;
+};
+
+template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
+    stencil_batch_kernel(const BatchEntry<T>* __restrict__ E, int nch) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int c = 0;
+  while (c + 1 < nch && E[c + 1].blk0 <= (int)blockIdx.x) ++c;
+  stencil_body<T, HELICAL, HAS_PREV, HAS_KB>(E[c].P, blockIdx.x - E[c].blk0, E[c].nblk,
+                                             reinterpret_cast<T*>(smem_raw));
+}
+
 // ---------------------------------------------------------------- KB gather
 // C[s][p] = gamma_p * sum_e g[e] * Y[s][node[e]]   (src/h_nonlocal_vector_mult.m:30-38)
 template <class T>
@@ -345,6 +374,41 @@ __global__ void __launch_bounds__(256) kb_gather_kernel(const T* __restrict__ Y,
     __syncthreads();
   }
   if (threadIdx.x == 0) C[(int64_t)s * nproj + p] = rmul(gamma[p], red[0]);
+}
+
+// Batched gather: entry c owns blocks [blk0, blk0 + nproj * ncol)
+template <class T>
+struct GatherEntry {
+  const T* Y;
+  int64_t ld;
+  const int* g_ptr;
+  const int* g_node;
+  const T* g_coef;
+  const typename traits<T>::real* gamma;
+  T* C;
+  int nproj, blk0;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_gather_batch_kernel(const GatherEntry<T>* __restrict__ E, int nch) {
+  __shared__ T red[256];
+  int c = 0;
+  while (c + 1 < nch && E[c + 1].blk0 <= (int)blockIdx.x) ++c;
+  const GatherEntry<T>& G = E[c];
+  const int loc = blockIdx.x - G.blk0;
+  const int p = loc % G.nproj;
+  const int s = loc / G.nproj;
+  const T* y = G.Y + (int64_t)s * G.ld;
+  T acc = zero<T>();
+  const int e1 = G.g_ptr[p + 1];
+  for (int e = G.g_ptr[p] + threadIdx.x; e < e1; e += blockDim.x) acc = cfma(G.g_coef[e], y[G.g_node[e]], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) G.C[(int64_t)s * G.nproj + p] = rmul(G.gamma[p], red[0]);
 }
 
 // ---------------------------------------------------------------- launch
@@ -389,28 +453,24 @@ static int choose_tr(int nr, int nt, int maxthr) {
   return tr;
 }
 
-template <class T, bool HEL, bool PREV, bool KB>
-static void launch_variant(const StepParams<T>& P, int nblk, int nthr, size_t shm, hipStream_t st) {
-  hipLaunchKernelGGL((stencil_step_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
-}
-
 template <class T>
-static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out_, int ncol, int64_t ld, double alpha,
-                     double shift, double beta) {
+struct StepLaunch {
+  StepParams<T> P;
+  long nblk;
+  int nthr;
+  size_t shm;
+  bool hel, kb;
+};
+
+// Parameters of one recurrence step of one channel.  tr_force > 0 sets the
+// radial tile (batched launches use one tile for all channels).
+template <class T>
+static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out_, int ncol, int64_t ld, double alpha,
+                     double shift, double beta, int tr_force, int flags, StepLaunch<T>& Lc) {
   using R = typename traits<T>::real;
   const cyx_grid_desc& g = h->g;
   const OpCoef<R>& C = coef_of<R>(h);
   const KbTables<T>& K = kb_of<T>(h);
-  hipStream_t st = h->ctx->stream;
-  const int64_t N = h->N();
-  const bool has_kb = K.nproj > 0;
-  if (has_kb) {
-    CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
-    PhaseTimer tm(h->ctx, PH_KB_GATHER);
-    hipLaunchKernelGGL(kb_gather_kernel<T>, dim3(K.nproj, ncol), dim3(256), 0, st, (const T*)in_, ld, K.g_ptr,
-                       K.g_node, K.g_coef, K.gamma, K.nproj, (T*)h->kbC.p);
-    CYX_HIP(hipGetLastError());
-  }
   StepParams<T> P{};
   P.in = (const T*)in_;
   P.prev = (const T*)prev_;
@@ -421,12 +481,17 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.nz = g.nz;
   P.tr = choose_tr(g.nr, g.nt, stencil_max_threads<T>());
   if (h->tune_tr > 0) P.tr = std::max(2 * FDN, std::min(h->tune_tr, stencil_max_threads<T>() / g.nt));
+  if (tr_force > 0) P.tr = tr_force;
+  CYX_CHECK(P.tr >= 2 * FDN && P.tr * g.nt <= stencil_max_threads<T>(), "radial tile %d does not fit (nt=%d)", P.tr,
+            g.nt);
   P.ntiles_r = ceil_div(g.nr, P.tr);
   int zb = g.nz;
   while ((long)P.ntiles_r * ceil_div(g.nz, zb) * ncol < 512 && zb > 4) zb = (zb + 1) / 2;
   if (h->tune_zb > 0) zb = std::min(h->tune_zb, g.nz);
   P.zb = zb;
-  P.nontemporal = (h->tune_flags & 1) ? 1 : 0;
+  P.nontemporal = (flags & 1) ? 1 : 0;
+  P.tile_major = (flags & 4) ? 1 : 0;
+  P.ncol = ncol;
   P.nblocks_z = ceil_div(g.nz, zb);
   P.alpha = (R)alpha;
   P.beta = (R)beta;
@@ -447,7 +512,7 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.ph_tr = phase_of<T>(h->ph[1][0], h->ph[1][1]);
   P.ph_zl = phase_of<T>(h->ph[2][0], h->ph[2][1]);
   P.ph_zr = phase_of<T>(h->ph[3][0], h->ph[3][1]);
-  P.s_ptr = K.s_ptr;
+  P.s_pk = K.s_pk;
   P.s_chi = K.s_chi;
   P.s_il = K.s_il;
   P.il_phase = K.il_phase;
@@ -460,28 +525,65 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.trh = P.tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
   const long nblk = (long)P.ntiles_r * P.nblocks_z * ncol;
   CYX_CHECK(nblk < (1L << 31), "grid too large");
-  P.xcd_remap = (nblk % 8 == 0 && !(h->tune_flags & 2)) ? 1 : 0;
-  const int nthr = P.tr * g.nt;
-  const size_t shm = (2 * (size_t)P.trh * (g.nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T);
-  const bool hel = g.cell_typ == 4 || g.cell_typ == 5;
-  const bool prev = prev_ != nullptr;
-  (void)N;
-#define CYX_LAUNCH(HEL, PRV, KBV) launch_variant<T, HEL, PRV, KBV>(P, (int)nblk, nthr, shm, st)
-  PhaseTimer tm(h->ctx, prev ? PH_STENCIL : PH_HAPPLY);
-  if (hel) {
+  P.xcd_remap = (nblk % 8 == 0 && !(flags & 2)) ? 1 : 0;
+  Lc.P = P;
+  Lc.nblk = nblk;
+  Lc.nthr = P.tr * g.nt;
+  Lc.shm = (2 * (size_t)P.trh * (g.nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T);
+  Lc.hel = g.cell_typ == 4 || g.cell_typ == 5;
+  Lc.kb = K.nproj > 0;
+  return CYX_OK;
+}
+
+template <class T, bool HEL, bool PREV, bool KB>
+static void launch_variant(const StepParams<T>& P, int nblk, int nthr, size_t shm, hipStream_t st) {
+  hipLaunchKernelGGL((stencil_step_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
+}
+
+template <class T>
+static void launch_single(const StepLaunch<T>& L, bool prev, hipStream_t st) {
+#define CYX_LAUNCH(HEL, PRV, KBV) launch_variant<T, HEL, PRV, KBV>(L.P, (int)L.nblk, L.nthr, L.shm, st)
+  if (L.hel) {
     if (prev) {
-      if (has_kb) CYX_LAUNCH(true, true, true); else CYX_LAUNCH(true, true, false);
+      if (L.kb) CYX_LAUNCH(true, true, true); else CYX_LAUNCH(true, true, false);
     } else {
-      if (has_kb) CYX_LAUNCH(true, false, true); else CYX_LAUNCH(true, false, false);
+      if (L.kb) CYX_LAUNCH(true, false, true); else CYX_LAUNCH(true, false, false);
     }
   } else {
     if (prev) {
-      if (has_kb) CYX_LAUNCH(false, true, true); else CYX_LAUNCH(false, true, false);
+      if (L.kb) CYX_LAUNCH(false, true, true); else CYX_LAUNCH(false, true, false);
     } else {
-      if (has_kb) CYX_LAUNCH(false, false, true); else CYX_LAUNCH(false, false, false);
+      if (L.kb) CYX_LAUNCH(false, false, true); else CYX_LAUNCH(false, false, false);
     }
   }
 #undef CYX_LAUNCH
+}
+
+template <class T>
+static int gather_launch(cyx_ham_s* h, const void* in_, int ncol, int64_t ld, hipStream_t st) {
+  const KbTables<T>& K = kb_of<T>(h);
+  if (K.nproj == 0) return CYX_OK;
+  CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
+  hipLaunchKernelGGL(kb_gather_kernel<T>, dim3(K.nproj, ncol), dim3(256), 0, st, (const T*)in_, ld, K.g_ptr,
+                     K.g_node, K.g_coef, K.gamma, K.nproj, (T*)h->kbC.p);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+template <class T>
+static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out_, int ncol, int64_t ld, double alpha,
+                     double shift, double beta) {
+  hipStream_t st = h->ctx->stream;
+  int rc;
+  {
+    PhaseTimer tm(h->ctx, PH_KB_GATHER);
+    if ((rc = gather_launch<T>(h, in_, ncol, ld, st)) != CYX_OK) return rc;
+  }
+  StepLaunch<T> L;
+  if ((rc = make_step<T>(h, in_, prev_, out_, ncol, ld, alpha, shift, beta, 0, h->tune_flags, L)) != CYX_OK)
+    return rc;
+  PhaseTimer tm(h->ctx, prev_ ? PH_STENCIL : PH_HAPPLY);
+  launch_single<T>(L, prev_ != nullptr, st);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
@@ -502,4 +604,336 @@ int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void
   return CYX_ERR_ARG;
 }
 
+// ---------------------------------------------------------------- batched filter plans
+template <class T, bool HEL, bool PREV, bool KB>
+static void launch_batch_variant(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, hipStream_t st) {
+  hipLaunchKernelGGL((stencil_batch_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, E, nch);
+}
+
+template <class T>
+static void launch_batch(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
+                         hipStream_t st) {
+#define CYX_BL(HEL, PRV, KBV) launch_batch_variant<T, HEL, PRV, KBV>(E, nch, nblk, nthr, shm, st)
+  if (hel) {
+    if (prev) {
+      if (kb) CYX_BL(true, true, true); else CYX_BL(true, true, false);
+    } else {
+      if (kb) CYX_BL(true, false, true); else CYX_BL(true, false, false);
+    }
+  } else {
+    if (prev) {
+      if (kb) CYX_BL(false, true, true); else CYX_BL(false, true, false);
+    } else {
+      if (kb) CYX_BL(false, false, true); else CYX_BL(false, false, false);
+    }
+  }
+#undef CYX_BL
+}
+
 }  // namespace cyx
+
+using namespace cyx;
+
+struct cyx_filter_plan_s {
+  cyx_ctx_s* ctx = nullptr;
+  int dtype = 0, nch = 0, m = 0;
+  std::vector<cyx_ham_s*> hams;
+  std::vector<void*> A, B;
+  std::vector<int> ncol;
+  std::vector<int64_t> ld;
+  int tr = 0, flags = 0;
+  bool hel = false, kb = false;
+  int nblk = 0, nthr = 0, gblk = 0;
+  size_t shm = 0;
+  void* d_steps = nullptr;   // [m][nch] BatchEntry<T>
+  void* d_gather = nullptr;  // [2][nch] GatherEntry<T>: inputs A, B
+  size_t step_bytes = 0, gather_bytes = 0;
+  std::vector<unsigned char> h_steps;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;  // capture stream (the context's stream may be the null stream)
+};
+
+namespace {
+
+template <class T>
+int plan_fill_steps(cyx_filter_plan_s* pl, const double* a, const double* b, const double* a0) {
+  const int nch = pl->nch, m = pl->m;
+  pl->h_steps.assign((size_t)m * nch * sizeof(BatchEntry<T>), 0);
+  BatchEntry<T>* E = reinterpret_cast<BatchEntry<T>*>(pl->h_steps.data());
+  for (int c = 0; c < nch; ++c) {
+    CYX_CHECK(b[c] > a[c] && a0[c] < a[c], "channel %d: filter needs a0 < a < b (a0=%g a=%g b=%g)", c, a0[c], a[c],
+              b[c]);
+    // Chebyshev scalars of src/chebyshev_filter.m:24-42
+    const double e = (b[c] - a[c]) / 2, cc = (b[c] + a[c]) / 2;
+    double sigma = e / (a0[c] - cc);
+    const double gamma = 2 / sigma;
+    void* X = pl->A[c];
+    void* Y = pl->B[c];
+    int blk0 = 0;
+    for (int i = 1; i <= m; ++i) {
+      StepLaunch<T> L;
+      int rc;
+      if (i == 1) {
+        rc = make_step<T>(pl->hams[c], X, nullptr, Y, pl->ncol[c], pl->ld[c], sigma / e, cc, 0.0, pl->tr, pl->flags, L);
+      } else {
+        const double sigma2 = 1 / (gamma - sigma);
+        rc = make_step<T>(pl->hams[c], Y, X, X, pl->ncol[c], pl->ld[c], 2 * sigma2 / e, cc, sigma * sigma2, pl->tr,
+                          pl->flags, L);
+        std::swap(X, Y);
+        sigma = sigma2;
+      }
+      if (rc != CYX_OK) return rc;
+      L.P.xcd_remap = 0;  // blocks of a channel do not start on an XCD boundary in general
+      BatchEntry<T>& be = E[(size_t)(i - 1) * nch + c];
+      be.P = L.P;
+      be.nblk = (int)L.nblk;
+      // channel offsets (same for every step)
+      if (c == 0) be.blk0 = 0;
+      else be.blk0 = E[(size_t)(i - 1) * nch + c - 1].blk0 + E[(size_t)(i - 1) * nch + c - 1].nblk;
+      blk0 = be.blk0 + be.nblk;
+      if (i == 1 && c == nch - 1) pl->nblk = blk0;
+    }
+  }
+  return CYX_OK;
+}
+
+template <class T>
+int plan_setup(cyx_filter_plan_s* pl) {
+  const int nch = pl->nch;
+  int nt = pl->hams[0]->g.nt;
+  pl->hel = false;
+  pl->kb = false;
+  size_t shm = 0;
+  int gblk = 0;
+  std::vector<GatherEntry<T>> G(2 * (size_t)nch);
+  for (int c = 0; c < nch; ++c) {
+    cyx_ham_s* h = pl->hams[c];
+    CYX_CHECK(h->g.nt == nt, "batched channels must share nt (%d vs %d)", h->g.nt, nt);
+    const bool hel = h->g.cell_typ == 4 || h->g.cell_typ == 5;
+    const KbTables<T>& K = kb_of<T>(h);
+    if (c == 0) {
+      pl->hel = hel;
+      pl->kb = K.nproj > 0;
+    }
+    CYX_CHECK(hel == pl->hel && (K.nproj > 0) == pl->kb,
+              "batched channels must all be helical or all cyclic, and all with or without projectors");
+    if (pl->kb)
+      CYX_CHECK(ensure(h->kbC, (size_t)pl->ncol[c] * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
+    const size_t trh = (size_t)pl->tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
+    shm = std::max(shm, (2 * trh * (nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T));
+    for (int v = 0; v < 2; ++v) {
+      GatherEntry<T>& g = G[(size_t)v * nch + c];
+      g.Y = (const T*)(v == 0 ? pl->A[c] : pl->B[c]);
+      g.ld = pl->ld[c];
+      g.g_ptr = K.g_ptr;
+      g.g_node = K.g_node;
+      g.g_coef = K.g_coef;
+      g.gamma = K.gamma;
+      g.C = (T*)h->kbC.p;
+      g.nproj = K.nproj;
+      g.blk0 = gblk;
+    }
+    gblk += K.nproj * pl->ncol[c];
+  }
+  pl->gblk = gblk;
+  pl->shm = shm;
+  pl->nthr = pl->tr * nt;
+  pl->step_bytes = (size_t)pl->m * nch * sizeof(BatchEntry<T>);
+  pl->gather_bytes = 2 * (size_t)nch * sizeof(GatherEntry<T>);
+  if (!pl->d_steps) {
+    CYX_HIP(hipMalloc(&pl->d_steps, pl->step_bytes));
+    CYX_HIP(hipMalloc(&pl->d_gather, pl->gather_bytes));
+  }
+  CYX_HIP(hipMemcpy(pl->d_gather, G.data(), pl->gather_bytes, hipMemcpyHostToDevice));
+  return CYX_OK;
+}
+
+// the m gathers + stencil steps of one filter pass, enqueued on st
+template <class T>
+void plan_enqueue(cyx_filter_plan_s* pl, hipStream_t st, int steps) {
+  const BatchEntry<T>* E = (const BatchEntry<T>*)pl->d_steps;
+  const GatherEntry<T>* G = (const GatherEntry<T>*)pl->d_gather;
+  // stencil input of step i: A for odd i, B for even i
+  for (int i = 1; i <= steps; ++i) {
+    if (pl->kb)
+      hipLaunchKernelGGL(kb_gather_batch_kernel<T>, dim3(pl->gblk), dim3(256), 0, st, G + ((i & 1) ? 0 : pl->nch),
+                         pl->nch);
+    launch_batch<T>(E + (size_t)(i - 1) * pl->nch, pl->nch, pl->nblk, pl->nthr, pl->shm, pl->hel, i > 1, pl->kb, st);
+  }
+}
+
+template <class T>
+int plan_run(cyx_filter_plan_s* pl, const double* a, const double* b, const double* a0) {
+  int rc;
+  if ((rc = plan_fill_steps<T>(pl, a, b, a0)) != CYX_OK) return rc;
+  hipStream_t st = pl->ctx->stream;
+  CYX_HIP(hipMemcpyAsync(pl->d_steps, pl->h_steps.data(), pl->step_bytes, hipMemcpyHostToDevice, st));
+  if (!pl->exec) {
+    // capture the pass once, on a stream of the plan's own (capture is not
+    // allowed on the null stream); later passes replay it with new scalars
+    if (!pl->cap) CYX_HIP(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    CYX_HIP(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+    plan_enqueue<T>(pl, pl->cap, pl->m);
+    hipError_t e = hipStreamEndCapture(pl->cap, &pl->graph);
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return CYX_ERR_HIP;
+    }
+    CYX_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  }
+  PhaseTimer tm(pl->ctx, PH_STENCIL);
+  CYX_HIP(hipGraphLaunch(pl->exec, st));
+  return CYX_OK;
+}
+
+// time one first-order step (reads A, writes B) per candidate (tile, flags)
+template <class T>
+int plan_autotune(cyx_filter_plan_s* pl, const double* a, const double* b, const double* a0) {
+  const int nt = pl->hams[0]->g.nt;
+  std::vector<int> trs;
+  for (int tr : {20, 24, 30, 16, 21, 15, 12})
+    if (tr >= 2 * FDN && tr * nt <= stencil_max_threads<T>() &&
+        std::find(trs.begin(), trs.end(), tr) == trs.end())
+      trs.push_back(tr);
+  hipStream_t st = pl->ctx->stream;
+  hipEvent_t e0, e1;
+  CYX_HIP(hipEventCreate(&e0));
+  CYX_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_tr = pl->tr, best_fl = pl->flags;
+  const int saved_m = pl->m;
+  pl->m = 1;
+  int rc = CYX_OK;
+  for (int tr : trs)
+    for (int fl : {0, 1, 4}) {
+      pl->tr = tr;
+      pl->flags = fl;
+      if ((rc = plan_setup<T>(pl)) != CYX_OK) break;
+      if ((rc = plan_fill_steps<T>(pl, a, b, a0)) != CYX_OK) break;
+      CYX_HIP(hipMemcpyAsync(pl->d_steps, pl->h_steps.data(), pl->step_bytes, hipMemcpyHostToDevice, st));
+      plan_enqueue<T>(pl, st, 1);
+      CYX_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 3; ++r) plan_enqueue<T>(pl, st, 1);
+      CYX_HIP(hipEventRecord(e1, st));
+      CYX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      CYX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tr = tr;
+        best_fl = fl;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pl->m = saved_m;
+  pl->tr = best_tr;
+  pl->flags = best_fl;
+  if (rc != CYX_OK) return rc;
+  hipFree(pl->d_steps);
+  pl->d_steps = nullptr;
+  hipFree(pl->d_gather);
+  pl->d_gather = nullptr;
+  return plan_setup<T>(pl);
+}
+
+template <class T>
+int plan_create(cyx_filter_plan_s* pl, int autotune, const double* a, const double* b, const double* a0) {
+  pl->tr = choose_tr(pl->hams[0]->g.nr, pl->hams[0]->g.nt, stencil_max_threads<T>());
+  pl->flags = 0;
+  int rc;
+  if (autotune) {
+    const int saved_m = pl->m;
+    pl->m = 1;
+    rc = plan_setup<T>(pl);
+    pl->m = saved_m;
+    if (rc != CYX_OK) return rc;
+    // the d_steps of the single-step setup is replaced after tuning
+    if ((rc = plan_autotune<T>(pl, a, b, a0)) != CYX_OK) return rc;
+    return CYX_OK;
+  }
+  return plan_setup<T>(pl);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cyx_filter_plan_create(cyx_ctx_t ctx, int nch, const cyx_ham_t* hams, int dtype, void* const* A, void* const* B,
+                           const int* ncol, const int64_t* ld, int m, int autotune, const double* a, const double* b,
+                           const double* a0, cyx_filter_plan_t* out) {
+  CYX_CHECK(ctx && hams && A && B && ncol && ld && out && nch >= 1 && m >= 1, "bad argument");
+  CYX_CHECK(!autotune || (a && b && a0), "autotuning needs filter bounds");
+  CYX_CHECK(dtype == CYX_F64 || dtype == CYX_C128 || dtype == CYX_F32 || dtype == CYX_C64, "bad dtype");
+  cyx_filter_plan_s* pl = new cyx_filter_plan_s();
+  pl->ctx = ctx;
+  pl->dtype = dtype;
+  pl->nch = nch;
+  pl->m = m;
+  for (int c = 0; c < nch; ++c) {
+    if (!hams[c] || !A[c] || !B[c] || A[c] == B[c] || ncol[c] < 1 || ld[c] < hams[c]->N() || hams[c]->ctx != ctx ||
+        (hams[c]->bloch && (dtype == CYX_F64 || dtype == CYX_F32))) {
+      delete pl;
+      set_error("channel %d: bad handle, buffers or shape", c);
+      return CYX_ERR_ARG;
+    }
+    pl->hams.push_back(hams[c]);
+    pl->A.push_back(A[c]);
+    pl->B.push_back(B[c]);
+    pl->ncol.push_back(ncol[c]);
+    pl->ld.push_back(ld[c]);
+  }
+  int rc;
+  switch (dtype) {
+    case CYX_F64: rc = plan_create<double>(pl, autotune, a, b, a0); break;
+    case CYX_F32: rc = plan_create<float>(pl, autotune, a, b, a0); break;
+    case CYX_C128: rc = plan_create<cplx<double>>(pl, autotune, a, b, a0); break;
+    default: rc = plan_create<cplx<float>>(pl, autotune, a, b, a0); break;
+  }
+  if (rc != CYX_OK) {
+    (void)hipFree(pl->d_steps);
+    (void)hipFree(pl->d_gather);
+    delete pl;
+    return rc;
+  }
+  *out = pl;
+  return CYX_OK;
+}
+
+int cyx_filter_plan_run(cyx_filter_plan_t pl, const double* a, const double* b, const double* a0, int* result_in_b) {
+  CYX_CHECK(pl && a && b && a0, "null argument");
+  int rc;
+  switch (pl->dtype) {
+    case CYX_F64: rc = plan_run<double>(pl, a, b, a0); break;
+    case CYX_F32: rc = plan_run<float>(pl, a, b, a0); break;
+    case CYX_C128: rc = plan_run<cplx<double>>(pl, a, b, a0); break;
+    default: rc = plan_run<cplx<float>>(pl, a, b, a0); break;
+  }
+  if (result_in_b) *result_in_b = (pl->m % 2) ? 1 : 0;
+  return rc;
+}
+
+int cyx_filter_plan_tuning(cyx_filter_plan_t pl, cyx_tuning* t) {
+  CYX_CHECK(pl && t, "null argument");
+  t->tr = pl->tr;
+  t->zb = 0;
+  t->flags = pl->flags;
+  t->reserved = 0;
+  return CYX_OK;
+}
+
+int cyx_filter_plan_destroy(cyx_filter_plan_t pl) {
+  if (!pl) return CYX_OK;
+  (void)hipDeviceSynchronize();  // the context may already be gone
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->cap) (void)hipStreamDestroy(pl->cap);
+  (void)hipFree(pl->d_steps);
+  (void)hipFree(pl->d_gather);
+  delete pl;
+  return CYX_OK;
+}
+
+}  // extern "C"
+

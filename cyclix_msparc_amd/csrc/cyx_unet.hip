@@ -12,8 +12,9 @@
 //
 // conv3: 3x3x3, stride 1, zero padding in r (the Dirichlet end), periodic in
 // theta and z; down: 2x2x2 stride 2; up: its transpose.  No activations, so
-// U is a linear operator.  In AAR the preconditioned residual is
-// D^{-1} r + U(D^{-1} r) (D the Jacobi diagonal of Lap_std).
+// U is a linear operator.  AAR uses d = D^{-1} r + U(D^{-1} r) (D the Jacobi
+// diagonal of Lap_std) as a correction direction with exact line search after
+// its Anderson iterations (cyx_aar.hip).
 //
 // Training: per-sample normalised squared error of U against the target,
 // backward through the same kernels (the adjoint of conv3 is conv3 with the
@@ -48,35 +49,47 @@ __device__ __forceinline__ int wrapn(int a, int n) { return a < 0 ? a + n : (a >
 
 // ---------------------------------------------------------------- FMA convolution (any mode, small layers)
 // out[b][q][co] (=|+=) sum W[tap][co][ci] in[b][p(q, tap)][ci] (+ skip[b][q][co])
-// One thread per output point and all output channels (Cout <= 64).
-template <int MODE, int COUT>
+// One thread per (output point, group of COB output channels); the weights
+// are staged in LDS as [tap][ci][co] so that a thread's COB weights of one
+// (tap, ci) are contiguous (wide broadcast reads).
+template <int MODE, int COB>
 __global__ void __launch_bounds__(256) conv_fma_kernel(const float* __restrict__ in, int cin, Lvl li,
-                                                       float* __restrict__ out, Lvl lo,
+                                                       float* __restrict__ out, int cout, Lvl lo,
                                                        const unsigned short* __restrict__ W,
                                                        const float* __restrict__ skip, int accumulate, int nb) {
-  extern __shared__ unsigned short wsh[];
+  extern __shared__ __align__(16) unsigned short wsh[];
   constexpr int NT = MODE == M_CONV3 ? 27 : 8;
-  const int nw = NT * COUT * cin;
-  for (int t = threadIdx.x; t < nw; t += blockDim.x) wsh[t] = W[t];
+  // per-tap stride padded by 16 elements (32 B): in up mode the lanes of a
+  // wave read up to 8 taps at once, which then fall in different banks
+  const int tstride = cin * cout + 16;
+  for (int tap = 0; tap < NT; ++tap)
+    for (int t = threadIdx.x; t < cin * cout; t += blockDim.x) {
+      const int co = t / cin, ci = t - co * cin;
+      wsh[tap * tstride + ci * cout + co] = W[tap * cin * cout + t];
+    }
   __syncthreads();
+  const int ncb = cout / COB;
   const int64_t no = npts(lo), ni = npts(li);
   const int64_t gq = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gq >= no * nb) return;
-  const int b = (int)(gq / no);
-  const int q = (int)(gq - (int64_t)b * no);
+  if (gq >= no * nb * ncb) return;
+  const int cb = (int)(gq % ncb);
+  const int64_t rest = gq / ncb;
+  const int b = (int)(rest / no);
+  const int q = (int)(rest - (int64_t)b * no);
   const int i = q % lo.nr, j = (q / lo.nr) % lo.nt, k = q / (lo.nr * lo.nt);
-  float acc[COUT];
+  float acc[COB];
 #pragma unroll
-  for (int c = 0; c < COUT; ++c) acc[c] = 0.f;
+  for (int c = 0; c < COB; ++c) acc[c] = 0.f;
   const float* ib = in + (int64_t)b * ni * cin;
+  const unsigned short* wc = wsh + cb * COB;
   if constexpr (MODE == M_UP) {
     const int tap = (k & 1) * 4 + (j & 1) * 2 + (i & 1);
     const int p = ((k >> 1) * li.nt + (j >> 1)) * li.nr + (i >> 1);
     for (int ci = 0; ci < cin; ++ci) {
       const float v = ib[(int64_t)p * cin + ci];
-      const unsigned short* w = wsh + (tap * COUT) * cin + ci;
+      const unsigned short* w = wc + tap * tstride + ci * cout;
 #pragma unroll
-      for (int c = 0; c < COUT; ++c) acc[c] = __builtin_fmaf(bf2f(w[c * cin]), v, acc[c]);
+      for (int c = 0; c < COB; ++c) acc[c] = __builtin_fmaf(bf2f(w[c]), v, acc[c]);
     }
   } else {
     for (int tap = 0; tap < NT; ++tap) {
@@ -94,20 +107,19 @@ __global__ void __launch_bounds__(256) conv_fma_kernel(const float* __restrict__
       const int p = (kk * li.nt + jj) * li.nr + ii;
       for (int ci = 0; ci < cin; ++ci) {
         const float v = ib[(int64_t)p * cin + ci];
-        const unsigned short* w = wsh + (tap * COUT) * cin + ci;
+        const unsigned short* w = wc + tap * tstride + ci * cout;
 #pragma unroll
-        for (int c = 0; c < COUT; ++c) acc[c] = __builtin_fmaf(bf2f(w[c * cin]), v, acc[c]);
+        for (int c = 0; c < COB; ++c) acc[c] = __builtin_fmaf(bf2f(w[c]), v, acc[c]);
       }
     }
   }
-  float* ob = out + ((int64_t)b * no + q) * COUT;
-  const float* sb = skip ? skip + ((int64_t)b * no + q) * COUT : nullptr;
+  const int64_t ob = ((int64_t)b * no + q) * cout + cb * COB;
 #pragma unroll
-  for (int c = 0; c < COUT; ++c) {
+  for (int c = 0; c < COB; ++c) {
     float v = acc[c];
-    if (sb) v += sb[c];
-    if (accumulate) v += ob[c];
-    ob[c] = v;
+    if (skip) v += skip[ob + c];
+    if (accumulate) v += out[ob + c];
+    out[ob + c] = v;
   }
 }
 
@@ -120,9 +132,12 @@ template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) conv3_mfma_kernel(const float* __restrict__ in, float* __restrict__ out, Lvl L,
                                                          const unsigned short* __restrict__ W,
                                                          const float* __restrict__ skip, int accumulate) {
+  // block: 4 waves on 4 consecutive theta rows; grid.x: (r tile, output-channel tile)
+  constexpr int NTL = COUT / 16;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int i0 = blockIdx.x * 16;
+  const int t = blockIdx.x % NTL;
+  const int i0 = (blockIdx.x / NTL) * 16;
   const int j = blockIdx.y * 4 + wave;
   const int k = blockIdx.z % L.nz;
   const int b = blockIdx.z / L.nz;
@@ -130,17 +145,16 @@ __global__ void __launch_bounds__(256) conv3_mfma_kernel(const float* __restrict
   const int m = lane & 15, kg = lane >> 4;
   const int64_t np = npts(L);
   const float* ib = in + (int64_t)b * np * CIN;
-  constexpr int NTL = COUT / 16;
-  v4f acc[NTL];
-#pragma unroll
-  for (int t = 0; t < NTL; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
   const bool row_ok = i0 + m < L.nr;
+#pragma unroll 3
   for (int tap = 0; tap < 27; ++tap) {
     const int ii = i0 + m + tap % 3 - 1;
     const int jj = wrapn(j + (tap / 3) % 3 - 1, L.nt);
     const int kk = wrapn(k + tap / 9 - 1, L.nz);
     const bool ok = row_ok && ii >= 0 && ii < L.nr;
     const int64_t p = ((int64_t)kk * L.nt + jj) * L.nr + (ok ? ii : 0);
+    const unsigned short* wt = W + ((int64_t)(tap * COUT + t * 16 + m) * CIN + kg * 4);
 #pragma unroll
     for (int kb = 0; kb < CIN / 16; ++kb) {
       const int ci0 = kb * 16 + kg * 4;
@@ -151,32 +165,25 @@ __global__ void __launch_bounds__(256) conv3_mfma_kernel(const float* __restrict
       a[1] = (short)f2bf(v.y);
       a[2] = (short)f2bf(v.z);
       a[3] = (short)f2bf(v.w);
-#pragma unroll
-      for (int t = 0; t < NTL; ++t) {
-        const unsigned short* wp = W + ((int64_t)(tap * COUT + t * 16 + m) * CIN + ci0);
-        const uint2 wv = *reinterpret_cast<const uint2*>(wp);
-        v4s bb;
-        bb[0] = (short)(wv.x & 0xffff);
-        bb[1] = (short)(wv.x >> 16);
-        bb[2] = (short)(wv.y & 0xffff);
-        bb[3] = (short)(wv.y >> 16);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
-      }
+      const uint2 wv = *reinterpret_cast<const uint2*>(wt + kb * 16);
+      v4s bb;
+      bb[0] = (short)(wv.x & 0xffff);
+      bb[1] = (short)(wv.x >> 16);
+      bb[2] = (short)(wv.y & 0xffff);
+      bb[3] = (short)(wv.y >> 16);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int t = 0; t < NTL; ++t) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int i = i0 + kg * 4 + e;
-      if (i < L.nr) {
-        const int64_t q = ((int64_t)k * L.nt + j) * L.nr + i;
-        const int64_t o = ((int64_t)b * np + q) * COUT + t * 16 + m;
-        float v = acc[t][e];
-        if (skip) v += skip[o];
-        if (accumulate) v += out[o];
-        out[o] = v;
-      }
+  for (int e = 0; e < 4; ++e) {
+    const int i = i0 + kg * 4 + e;
+    if (i < L.nr) {
+      const int64_t q = ((int64_t)k * L.nt + j) * L.nr + i;
+      const int64_t o = ((int64_t)b * np + q) * COUT + t * 16 + m;
+      float v = acc[e];
+      if (skip) v += skip[o];
+      if (accumulate) v += out[o];
+      out[o] = v;
     }
   }
 }
@@ -462,22 +469,28 @@ template <int MODE>
 int launch_fma(cyx_unet_s* net, const float* in, int cin, Lvl li, float* out, int cout, Lvl lo,
                const unsigned short* W, const float* skip, int accumulate, int nb) {
   hipStream_t st = net->ctx->stream;
-  const int64_t tot = npts(lo) * nb;
-  const size_t shm = (size_t)ntap(MODE) * cin * cout * sizeof(unsigned short);
+  const size_t shm = (size_t)ntap(MODE) * (cin * cout + 16) * sizeof(unsigned short);
   if (shm > 64 * 1024) {
     set_error("internal: FMA convolution weights exceed LDS (%zu bytes)", shm);
     return CYX_ERR_ARG;
   }
-  const dim3 grid((unsigned)ceil_div(tot, 256));
-#define CYX_FMA(C) hipLaunchKernelGGL((conv_fma_kernel<MODE, C>), grid, dim3(256), shm, st, in, cin, li, out, lo, W, skip, accumulate, nb)
-  switch (cout) {
-    case 1: CYX_FMA(1); break;
-    case 16: CYX_FMA(16); break;
-    case 32: CYX_FMA(32); break;
-    case 64: CYX_FMA(64); break;
-    default: set_error("internal: unsupported channel count %d", cout); return CYX_ERR_ARG;
+  const int cob = cout >= 16 ? 16 : cout;
+  if (cout % cob) {
+    set_error("internal: unsupported channel count %d", cout);
+    return CYX_ERR_ARG;
   }
-#undef CYX_FMA
+  const int64_t tot = npts(lo) * nb * (cout / cob);
+  const dim3 grid((unsigned)ceil_div(tot, 256));
+  if (cob == 16)
+    hipLaunchKernelGGL((conv_fma_kernel<MODE, 16>), grid, dim3(256), shm, st, in, cin, li, out, cout, lo, W, skip,
+                       accumulate, nb);
+  else if (cob == 1)
+    hipLaunchKernelGGL((conv_fma_kernel<MODE, 1>), grid, dim3(256), shm, st, in, cin, li, out, cout, lo, W, skip,
+                       accumulate, nb);
+  else {
+    set_error("internal: unsupported channel count %d", cout);
+    return CYX_ERR_ARG;
+  }
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
@@ -486,7 +499,7 @@ int conv3(cyx_unet_s* net, const float* in, int cin, float* out, int cout, int l
           const float* skip, int accumulate, int nb) {
   const Lvl L = net->lv[lvl];
   hipStream_t st = net->ctx->stream;
-  const dim3 grid((unsigned)ceil_div(L.nr, 16), (unsigned)ceil_div(L.nt, 4), (unsigned)(L.nz * nb));
+  const dim3 grid((unsigned)(ceil_div(L.nr, 16) * (cout / 16)), (unsigned)ceil_div(L.nt, 4), (unsigned)(L.nz * nb));
   if (cin == 32 && cout == 32) {
     hipLaunchKernelGGL((conv3_mfma_kernel<32, 32>), grid, dim3(256), 0, st, in, out, L, W, skip, accumulate);
   } else if (cin == 64 && cout == 64) {
@@ -704,7 +717,7 @@ int cyx_unet_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_unet_t* out) {
 
 int cyx_unet_destroy(cyx_unet_t n) {
   if (!n) return CYX_OK;
-  (void)hipStreamSynchronize(n->ctx->stream);
+  (void)hipDeviceSynchronize();  // the context may already be gone
   (void)hipFree(n->p); (void)hipFree(n->g); (void)hipFree(n->m1); (void)hipFree(n->m2);
   (void)hipFree(n->wb); (void)hipFree(n->wt); (void)hipFree(n->red); (void)hipFree(n->act);
   delete n;

@@ -1,17 +1,20 @@
 """Learned part of the AAR Poisson preconditioner.
 
-A linear three-level 3-D convolutional U-Net U (cyx_unet.hip) makes the AAR
-preconditioner M r = D^{-1} r + U(D^{-1} r), D the Jacobi diagonal of
-Lap_std.  The plain Jacobi-AAR stays the fallback and the judge: every solve
-stops on the true relative residual ||f - Lap_std x|| / ||f|| of src/aar.m,
-whatever M is.
+A linear three-level 3-D convolutional U-Net U (cyx_unet.hip) approximates
+Lap_std^{-1}: d = D^{-1} r + U(D^{-1} r) (D the Jacobi diagonal) is the
+correction for a residual r.  Inside AAR it is applied after every Anderson
+iteration as x += delta d with the exact line search delta = <Lap d, r> /
+<Lap d, Lap d>, which can only lower the residual norm.  The plain
+Jacobi-AAR iteration stays the fallback and the judge: every solve stops on
+the true relative residual ||f - Lap_std x|| / ||f|| of src/aar.m.
 
-Training data are (rho, phi) pairs from the method's own solves: random
-densities of 4-12 Gaussians (sigma 0.5-2 Bohr, total charge 32 e), f =
--4 pi rho, phi = aar(Lap_std, f) to 1e-8 with the Jacobi preconditioner.
-The network is fitted to phi - D^{-1} f from the input D^{-1} f (each pair
-scaled by 1 / rms(D^{-1} f), which a linear map does not notice) with the
-device's own forward and backward kernels and Adam.
+Training data come from the method's own solves: random densities of 4-12
+Gaussians (sigma 0.5-2 Bohr, total charge 32 e), f = -4 pi rho, solved by
+Jacobi-AAR; each solve gives the pair (f, phi) and, from the same solve
+stopped early, residual/error pairs (r_K, e_K) with Lap_std e_K = r_K (see
+generate_pairs).  The network is fitted to e - D^{-1} r from the input
+D^{-1} r (each pair scaled by 1 / rms(D^{-1} r), which a linear map does not
+notice) with the device's own forward and backward kernels and Adam.
 """
 from __future__ import annotations
 
@@ -195,40 +198,65 @@ class DensitySampler:
         return rho * (self.q / float((rho * self.W).sum()))
 
 
-def generate_pairs(dev: Device, system, n, seed=0, tol=1e-8, max_iter=4000, log=None):
-    """n (f, phi) pairs, f = -4 pi rho, phi = Jacobi-AAR solution (FP64 on
-    the device, stored FP32): returns two (n, N) float32 device tensors and
-    the iteration counts."""
+def generate_pairs(dev: Device, system, n_densities, seed=0, tol=1e-6, max_iter=8000, snapshots=(),
+                   noise_fraction=0.0, log=None):
+    """Residual/error pairs from the method's own Jacobi-AAR solves.
+
+    For each random density (f = -4 pi rho) the solve to tol gives x*; AAR
+    runs stopped after K iterations (K in snapshots, cold start) give x_K,
+    and each (r_K, e_K) = (f - Lap_std x_K, x* - x_K) is a pair, as is
+    (f, x*).  These are the residuals the preconditioner sees inside AAR,
+    with the corrections that would solve them exactly.  A fraction of the
+    right-hand sides are white noise (pair (f, x*) only), so that the rough
+    end of the spectrum is represented too.  Pairs are stored FP32 (their
+    rounding is far below tol = 1e-6).  Returns (R, E) as (n, N) float32
+    device tensors and the Jacobi iteration counts of the full solves."""
     g = system.grid
     td = dev.torch_device
     ps = Poisson(dev, g)
-    F = torch.empty((n, g.N), dtype=torch.float32, device=td)
-    PHI = torch.empty_like(F)
-    its = np.zeros(n, dtype=int)
     rng = np.random.default_rng(seed)
     sampler = DensitySampler(system, td)
+    gen = torch.Generator(device=td)
+    gen.manual_seed(seed)
+    is_noise = np.zeros(n_densities, dtype=bool)
+    is_noise[rng.permutation(n_densities)[:int(round(noise_fraction * n_densities))]] = True
+    npairs = int(np.sum(np.where(is_noise, 1, 1 + len(snapshots))))
+    R = torch.empty((npairs, g.N), dtype=torch.float32, device=td)
+    E = torch.empty_like(R)
+    its = np.zeros(n_densities, dtype=int)
     t0 = time.time()
-    for k in range(n):
-        f = (-4 * np.pi * sampler.sample(rng)).contiguous()
-        phi, info = ps.aar(f, None, tol, max_iter)
+    row = 0
+    for k in range(n_densities):
+        if is_noise[k]:
+            f = torch.randn(g.N, dtype=torch.float64, device=td, generator=gen).contiguous()
+        else:
+            f = (-4 * np.pi * sampler.sample(rng)).contiguous()
+        xs, info = ps.aar(f, None, tol, max_iter)
         if not info.converged:
             raise RuntimeError(f"training solve {k} did not converge (relres {info.relres:.3e})")
-        F[k] = f.to(torch.float32)
-        PHI[k] = phi.to(torch.float32)
         its[k] = info.iterations
-        if log and (k + 1) % 256 == 0:
-            log(f"pairs {k + 1}/{n}: {time.time() - t0:.1f} s, mean AAR iterations {its[:k + 1].mean():.0f}")
+        R[row] = f.to(torch.float32)
+        E[row] = xs.to(torch.float32)
+        row += 1
+        if not is_noise[k]:
+            for K in snapshots:
+                xk, _ = ps.aar(f, None, 1e-300, K)
+                R[row] = (f - ps.apply(xk)).to(torch.float32)
+                E[row] = (xs - xk).to(torch.float32)
+                row += 1
+        if log and (k + 1) % 128 == 0:
+            log(f"densities {k + 1}/{n_densities}: {time.time() - t0:.1f} s, mean AAR iterations {its[:k + 1].mean():.0f}")
     ps.close()
-    return F, PHI, its
+    return R, E, its
 
 
-def training_tensors(unet: UNet, poisson: Poisson, F, PHI):
-    """Inputs D^{-1} f and targets phi - D^{-1} f, each pair scaled by
-    1 / rms(D^{-1} f)."""
-    X = unet.input_of(poisson, F.to(torch.float64).contiguous())
+def training_tensors(unet: UNet, poisson: Poisson, R, E):
+    """Inputs D^{-1} r and targets e - D^{-1} r, each pair scaled by
+    1 / rms(D^{-1} r)."""
+    X = unet.input_of(poisson, R.to(torch.float64).contiguous())
     scale = 1.0 / torch.sqrt(torch.mean(X.to(torch.float64) ** 2, dim=1))
     X = (X * scale[:, None].to(torch.float32)).contiguous()
-    T = (PHI * scale[:, None].to(torch.float32) - X).contiguous()
+    T = (E * scale[:, None].to(torch.float32) - X).contiguous()
     return X, T
 
 
@@ -273,7 +301,8 @@ def default_weights_path(grid: GridSpec) -> Path:
     return WEIGHTS_DIR / f"unet_{grid.nr}x{grid.nt}x{grid.nz}.npz"
 
 
-def compare_solves(dev: Device, system, unet: UNet, n=16, seed=12345, tol=1e-8, max_iter=4000):
+def compare_solves(dev: Device, system, unet: UNet, n=16, seed=12345, tol=1e-8, max_iter=4000, omega=0.6,
+                   beta=0.6, p_learned=6):
     """Jacobi-AAR vs learned-preconditioner AAR on n fresh random densities
     (cold start, f = -4 pi rho): per-solve iterations and milliseconds."""
     g = system.grid
@@ -287,13 +316,15 @@ def compare_solves(dev: Device, system, unet: UNet, n=16, seed=12345, tol=1e-8, 
         for mode in ("jacobi", "learned"):
             if mode == "learned":
                 unet.attach(ps)
-            ps.aar(f, None, tol, 20)            # warm the launch path
+            pp = p_learned if mode == "learned" else 6
+            ps.aar(f, None, tol, 20, omega, beta, 7, pp)   # warm the launch path
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, info = ps.aar(f, None, tol, max_iter)
+            x, info = ps.aar(f, None, tol, max_iter, omega, beta, 7, pp)
             torch.cuda.synchronize()
             row[mode] = {"iterations": int(info.iterations), "ms": (time.perf_counter() - t0) * 1e3,
-                         "relres": float(info.relres), "converged": bool(info.converged)}
+                         "relres": float(info.relres), "converged": bool(info.converged),
+                         "precond": int(info.precond)}
             if mode == "learned":
                 unet.detach(ps)
         out.append(row)
@@ -310,29 +341,37 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--workload", default="cnt10_helical")
-    ap.add_argument("--pairs", type=int, default=4096)
-    ap.add_argument("--val", type=int, default=128)
+    ap.add_argument("--densities", type=int, default=4096, help="solves; each gives 1 + len(snapshots) pairs")
+    ap.add_argument("--snapshots", default="", help="comma-separated AAR iteration counts for residual/error pairs")
+    ap.add_argument("--val", type=int, default=32, help="densities held out for validation")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--compare", type=int, default=16)
+    ap.add_argument("--noise", type=float, default=0.0, help="fraction of white-noise right-hand sides")
+    ap.add_argument("--omega", type=float, default=0.6)
+    ap.add_argument("--learned-p", type=int, default=6, help="AAR Anderson period with the learned correction")
+    ap.add_argument("--compare-maxit", type=int, default=4000)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
     system = make_system(args.workload)
     dev = Device(0)
     t0 = time.time()
-    F, PHI, its = generate_pairs(dev, system, args.pairs + args.val, seed=args.seed, log=log)
-    log(f"generated {args.pairs + args.val} pairs in {time.time() - t0:.1f} s (Jacobi-AAR iterations: mean "
-        f"{its.mean():.0f}, max {its.max()})")
+    snaps = tuple(int(v) for v in args.snapshots.split(",") if v)
+    F, PHI, its = generate_pairs(dev, system, args.densities, seed=args.seed, snapshots=snaps,
+                                 noise_fraction=args.noise, log=log)
+    Fv, PHIv, _ = generate_pairs(dev, system, args.val, seed=args.seed + 1, snapshots=snaps,
+                                 noise_fraction=args.noise)
+    log(f"generated {F.shape[0]} training and {Fv.shape[0]} validation pairs in {time.time() - t0:.1f} s "
+        f"(Jacobi-AAR iterations: mean {its.mean():.0f}, max {its.max()})")
     unet = UNet(dev, system.grid)
     unet.init_params(seed=args.seed)
     ps = Poisson(dev, system.grid)
-    X, T = training_tensors(unet, ps, F, PHI)
-    del F, PHI
-    Xt, Tt = X[:args.pairs].contiguous(), T[:args.pairs].contiguous()
-    Xv, Tv = X[args.pairs:].contiguous(), T[args.pairs:].contiguous()
+    Xt, Tt = training_tensors(unet, ps, F, PHI)
+    Xv, Tv = training_tensors(unet, ps, Fv, PHIv)
+    del F, PHI, Fv, PHIv
     log(f"validation loss before training: {evaluate(unet, Xv, Tv):.4e}")
     t1 = time.time()
     hist = train(unet, Xt, Tt, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, val=(Xv, Tv),
@@ -341,7 +380,16 @@ def main(argv=None):
     out = Path(args.out) if args.out else default_weights_path(system.grid)
     out.parent.mkdir(parents=True, exist_ok=True)
     unet.save(out)
-    cmp = compare_solves(dev, system, unet, n=args.compare) if args.compare > 0 else []
+    x1 = Xv[:1].contiguous()
+    unet.forward(x1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(20):
+        unet.forward(x1)
+    torch.cuda.synchronize()
+    log(f"forward (one sample): {(time.perf_counter() - t2) / 20 * 1e3:.3f} ms")
+    cmp = compare_solves(dev, system, unet, n=args.compare, max_iter=args.compare_maxit, omega=args.omega,
+                         beta=args.omega, p_learned=args.learned_p) if args.compare > 0 else []
     for row in cmp:
         log(json.dumps(row))
     if cmp:

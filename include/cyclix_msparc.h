@@ -79,6 +79,7 @@ typedef struct cyx_ham_s* cyx_ham_t;
 typedef struct cyx_poisson_s* cyx_poisson_t;
 typedef struct cyx_mixer_s* cyx_mixer_t;
 typedef struct cyx_unet_s* cyx_unet_t;
+typedef struct cyx_filter_plan_s* cyx_filter_plan_t;
 
 /* ------------------------------------------------------------ context */
 const char* cyx_last_error(void);
@@ -117,7 +118,9 @@ const double* cyx_ham_weights(cyx_ham_t ham);
  * counterpart in the reference).  tr: radial tile (0 = heuristic; clamped to
  * the kernel's thread limit), zb: z planes per workgroup (0 = heuristic),
  * flags: bit 0 non-temporal loads of X_prev and stores of the result, bit 1
- * disable the XCD-aware block remap.  Results do not depend on these. */
+ * disable the XCD-aware block remap, bit 2 tile-major block order (blocks of
+ * one radial tile adjacent, so an XCD's resident blocks share that tile's
+ * tables in L2).  Results do not depend on these. */
 typedef struct cyx_tuning {
   int32_t tr, zb, flags, reserved;
 } cyx_tuning;
@@ -154,6 +157,28 @@ int cyx_rayleigh_ritz(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, d
  * src/eigSolver.m:47-63 (tolerance TOL_LANCZOS, at most maxit steps). */
 int cyx_lanczos_bounds(cyx_ham_t ham, const double* v0, int maxit, double tol, double* lambda_min,
                        double* lambda_max);
+
+/* ------------------------------------------------------------ batched filter */
+/* Chebyshev filters of many channels at once: the per-channel loop of
+ * src/eigSolver.m:26-98 (chebyshev_filter on every (nu, eta) channel) as
+ * one launch per recurrence step over all channels, the m steps of a pass
+ * captured once into a HIP graph and replayed.  Channel c filters the
+ * ncol[c] columns of A[c] (device, leading dimension ld[c]) with B[c] as
+ * work; after a run the result is in B[c] when *result_in_b, else in A[c]
+ * (A[c] is overwritten either way).  All channels share dtype, nt, the
+ * helical/cyclic form and whether they have projectors; grids, Bloch phases
+ * and column counts may differ.  autotune != 0 times one step per candidate
+ * radial tile and block order on these buffers (using bounds a, b, a0) and
+ * keeps the fastest (cyx_filter_plan_tuning reports it).  Buffers and
+ * channels are fixed for the plan's life; bounds change per run. */
+int cyx_filter_plan_create(cyx_ctx_t ctx, int nch, const cyx_ham_t* hams, int dtype, void* const* A, void* const* B,
+                           const int* ncol, const int64_t* ld, int m, int autotune, const double* a, const double* b,
+                           const double* a0, cyx_filter_plan_t* out);
+/* one pass: bounds per channel (host arrays of nch), src/chebyshev_filter.m:24-42 */
+int cyx_filter_plan_run(cyx_filter_plan_t plan, const double* a, const double* b, const double* a0,
+                        int* result_in_b);
+int cyx_filter_plan_tuning(cyx_filter_plan_t plan, cyx_tuning* t);
+int cyx_filter_plan_destroy(cyx_filter_plan_t plan);
 
 /* ------------------------------------------------------------ SCF glue (FP64) */
 /* rho += 2 * weight * sum_s occ[s] |psi_s|^2 over the ns device states psi
@@ -221,6 +246,10 @@ typedef struct cyx_aar_info {
   int32_t iterations; /* count-1 at exit, as src/aar.m:394-397 reports */
   int32_t converged;  /* relres <= tol */
   double relres;      /* ||f - A x|| / ||f|| of the returned x */
+  int32_t precond;    /* 0 Jacobi; 1 learned (cyx_poisson_set_preconditioner);
+                         2 learned, diverged, solved again with Jacobi (iterations
+                         then count the Jacobi solve) */
+  int32_t reserved;
 } cyx_aar_info;
 
 /* Device operator for the Poisson problem: S.Lap_std at k = 0
@@ -243,14 +272,21 @@ int cyx_aar(cyx_poisson_t ps, const double* f, double* x, int use_guess, double 
 int cyx_poisson_solve(cyx_poisson_t ps, const double* rho, const double* b, double* phi, int is_guess, double tol,
                       int max_iter, cyx_aar_info* info);
 
+/* y = Lap_std x (device FP64, length N): the operator AAR solves with. */
+int cyx_poisson_apply(cyx_poisson_t ps, const double* x, double* y);
+
 /* ------------------------------------------------------------ learned Poisson preconditioner */
 /* A linear three-level 3-D convolutional U-Net U on the handle grid (nr, nt,
  * nz divisible by 4): BF16 weights, BF16 matrix-core products, FP32
- * accumulation.  With it, the AAR preconditioner of a Poisson handle becomes
- * M r = D^{-1} r + U(D^{-1} r), D the Jacobi diagonal of Lap_std, in place of
- * the reference's ILU factors (src/poissonSolve.m:36).  It has no counterpart
- * in the reference.  Parameters: one FP32 vector (cyx_unet_num_params
- * entries: the layer weights, then a per-radius output scale). */
+ * accumulation.  With it, every Anderson iteration of AAR on a Poisson
+ * handle is followed by a learned correction x += delta d with
+ * d = D^{-1} r + U(D^{-1} r) (r = f - Lap_std x, D the Jacobi diagonal) and
+ * delta = <Lap d, r> / <Lap d, Lap d>, the step that minimises the new
+ * residual norm; the Jacobi-AAR iteration itself and its stopping test on the
+ * true residual are unchanged (src/aar.m, src/poissonSolve.m:36, where the
+ * reference uses ILU factors instead).  It has no counterpart in the
+ * reference.  Parameters: one FP32 vector (cyx_unet_num_params entries: the
+ * layer weights, then a per-radius output scale). */
 int cyx_unet_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_unet_t* out);
 int cyx_unet_destroy(cyx_unet_t net);
 int64_t cyx_unet_num_params(cyx_unet_t net);

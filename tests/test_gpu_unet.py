@@ -102,37 +102,36 @@ def _poisson_case(device):
     return g, Poisson(device, g), f
 
 
-def test_learned_aar_with_zero_network_is_jacobi_aar(device):
-    """U = 0 (last layer zero): the learned path's iterates are the Jacobi
-    path's, bit for bit."""
+def test_learned_aar_with_zero_network_converges_to_jacobi_solution(device):
+    """U = 0 (last layer zero): the correction is a Jacobi step with exact
+    line search after each Anderson iteration; the solve reaches the Jacobi
+    solution and never needs more iterations."""
     from cyclix_msparc_amd.learned import UNet
     g, ps, f = _poisson_case(device)
-    x_j, info_j = ps.aar(f, None, 1e-8, 3000)
+    x_j, info_j = ps.aar(f, None, 1e-10, 3000)
     net = UNet(device, g)
     net.init_params(seed=1)
     net.attach(ps)
-    x_l, info_l = ps.aar(f, None, 1e-8, 3000)
-    assert info_j.converged and info_l.converged
-    assert info_l.iterations == info_j.iterations
-    assert torch.equal(x_l, x_j)
+    x_l, info_l = ps.aar(f, None, 1e-10, 3000)
+    assert info_j.converged and info_l.converged and info_l.precond == 1
+    assert info_l.iterations <= info_j.iterations
+    assert torch.max(torch.abs(x_l - x_j)) <= 1e-6 * torch.max(torch.abs(x_j))
     net.close()
     ps.close()
 
 
 def test_learned_aar_converges_with_random_network(device):
-    """Any U only changes the path: the solve still stops on the true
-    residual and reaches the Jacobi solution."""
+    """Any U only changes the path (the line search cannot raise the
+    residual): the solve still stops on the true residual and reaches the
+    Jacobi solution."""
     from cyclix_msparc_amd.learned import UNet
     g, ps, f = _poisson_case(device)
     x_j, info_j = ps.aar(f, None, 1e-10, 5000)
     net = UNet(device, g)
-    p = rand_params(g.nr, 7)
-    off, _ = U.offsets(g.nr)
-    p[:off[-1]] *= 0.3
-    net.set_params(p)
+    net.set_params(rand_params(g.nr, 7))
     net.attach(ps)
     x_l, info_l = ps.aar(f, None, 1e-10, 5000)
-    assert info_l.converged and info_l.relres <= 1e-10
+    assert info_l.converged and info_l.relres <= 1e-10 and info_l.precond == 1
     assert torch.max(torch.abs(x_l - x_j)) <= 1e-6 * torch.max(torch.abs(x_j))
     net.close()
     ps.close()
