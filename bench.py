@@ -4,7 +4,13 @@
 Workload: (10,0) carbon nanotube, 1/5 sector with helical twist 0.05 rad/Bohr,
 8 atoms (synthetic Gaussian local + s-KB pseudopotential), grid
 120 x 24 x 96 (N = 276,480), one complex Bloch channel (nu = 1,
-eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24, complex128 (FP64).
+eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24.
+
+Two paths are timed: the FP64 path (complex128 throughout, Jacobi-AAR) and
+the mixed path (complex64 psi, filter and H-applies with FP64 accumulation of
+the Gram products, FP64 dense eigensolve, density and energies; the learned
+Poisson correction when its weights for this grid are present).  `value` is
+the mixed path's time per step; `speedup_vs_fp64` compares the two.
 
 One step = one SCF iteration of src/scf.m on the device, at the SCF_tol of
 a 1e-5 Ha/atom energy target (src/initialization.m:516-552): CheFSI pass
@@ -17,6 +23,7 @@ density, cold start, 1e-8 relative residual) is timed separately.
 
 Prints one JSON line (metric, value in ms per SCF step, roofline of the
 dominant kernel from live HIP-event timing, CPU baseline from the oracle).
+`--paths fp64` times the FP64 path alone.
 """
 from __future__ import annotations
 
@@ -42,7 +49,10 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", default="cnt10_helical")
-    p.add_argument("--dtype", default="c128", choices=["c128", "c64"])
+    p.add_argument("--paths", default="fp64,mixed",
+                   help="comma list of fp64 (complex128, Jacobi-AAR) and mixed (complex64 filter and subspace "
+                        "inputs with FP64 accumulation, learned Poisson correction when weights exist)")
+    p.add_argument("--no-learned", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
@@ -52,37 +62,29 @@ def parse():
     return p.parse_args()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the channel sharding across GPUs "
-              "is described in DESIGN.md and not part of this build.", file=sys.stderr)
-        sys.exit(2)
+def run_path(dev, sysm, args, precision, unet=None):
+    """W warm-up and K timed SCF iterations of one path; returns its record."""
     import torch
 
-    from cyclix_msparc_amd.api import Device, Poisson
     from cyclix_msparc_amd.scf import GroundState, SCFOptions
-    from cyclix_msparc_amd.system import make_system
-
-    bsz = 16 if args.dtype == "c128" else 8
-    sysm = make_system(args.workload)
     g = sysm.grid
     N, ns, npl = g.N, sysm.nstates, sysm.npl
-    dev = Device(0)
-    tdev = dev.torch_device
+    bsz = 16 if precision == "fp64" else 8
     # SCF_tol for 1e-5 Ha/atom (src/initialization.m:516-545), poisson_tol = 0.01 SCF_tol;
-    # MINIT_SCF keeps the loop running through warmup + timed iterations
+    # MINIT_SCF keeps the loop running through warm-up + timed iterations
     opts = SCFOptions(target_energy_accuracy=1e-5, MINIT_SCF=args.warmup + args.steps,
                       MAXIT_SCF=max(100, args.warmup + args.steps), MAXIT_POISSON=args.aar_maxit)
-    gs = GroundState(dev, sysm, options=opts, precision="fp64" if args.dtype == "c128" else "fp32")
+    gs = GroundState(dev, sysm, options=opts, precision=precision)
+    if unet is not None:
+        unet.attach(gs.poisson)
     rng = np.random.default_rng(0)
     X = rng.standard_normal((ns, N)) + 1j * rng.standard_normal((ns, N))
-    if args.dtype == "c64":
+    if precision != "fp64":
         X = X.astype(np.complex64)
     gs.init_psi([X])
     del X
     it = gs.iterations()
-    # warmup: the first pass with its Lanczos bounds, the Chebyshev cycles
+    # warm-up: the first pass with its Lanczos bounds, the Chebyshev cycles
     # before rhoTrigger and `warmup` SCF iterations (src/scf.m:111-125)
     for _ in range(args.warmup):
         next(it)
@@ -101,8 +103,7 @@ def main():
     tm = dev.timing()
     dev.set_timing(False)
     it.close()
-
-    # roofline of the dominant kernel: fused stencil + three-term step
+    # roofline of the dominant kernel, the fused stencil + three-term step:
     # algorithmic bytes per launch = read Y + read X_prev + write Y_new (ns
     # columns of N points, bsz bytes) + Veff (N doubles, shared by columns)
     st_ms, st_n = tm["stencil"]
@@ -112,24 +113,83 @@ def main():
     # whole filter: B_filter = (3m-1) N ns b + m N 8; its time is the m-1
     # stencil launches, the first-step H-apply and the m KB gathers of the
     # filter (Rayleigh-Ritz adds one H-apply and one gather per pass)
-    ha_ms, ha_n = tm["h_apply"]
-    kb_ms, kb_n = tm["kb_gather"]
-    filt_ms_only = (st_ms + ha_ms / 2 + kb_ms * npl / (npl + 1)) / args.steps
+    ha_ms, _ = tm["h_apply"]
+    kb_ms, _ = tm["kb_gather"]
+    filt_ms = (st_ms + ha_ms / 2 + kb_ms * npl / (npl + 1)) / args.steps
     b_filter = (3 * npl - 1) * N * ns * bsz + npl * N * 8.0
-    filter_gbs = b_filter / (filt_ms_only * 1e-3) / 1e9
+    filter_gbs = b_filter / (filt_ms * 1e-3) / 1e9
+    rec = {
+        "ms_per_step": round(ms_step, 3),
+        "dtype": "complex128" if precision == "fp64" else "complex64 (FP64 accumulation, FP64 dense eig/energies)",
+        "learned_poisson": unet is not None,
+        "roofline": {"bound": "hbm", "kernel": f"stencil_step_kernel<{'c128' if precision == 'fp64' else 'c64'}> "
+                                               "(fused H-apply + Chebyshev step)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4)},
+        "filter": {"ms": round(filt_ms, 3), "GB/s": round(filter_gbs, 1),
+                   "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4)},
+        "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in tm.items()},
+        "scf": {"err": [round(r.err, 6) for r in recs], "Eatom": [round(r.Etotal / sysm.n_atm, 8) for r in recs],
+                "aar_iterations": [r.aar_iterations for r in recs]},
+    }
+    state = dict(bounds=gs.bounds, veff=gs.veff.cpu().numpy(),
+                 aar_its=float(np.mean([r.aar_iterations for r in recs])))
+    if unet is not None:
+        unet.detach(gs.poisson)
+    del gs
+    torch.cuda.empty_cache()
+    return rec, state
 
-    # the Poisson solve of the workload on its own: rho = sum of atom-centred
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the channel sharding across GPUs "
+              "is described in DESIGN.md and not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from cyclix_msparc_amd.api import Device, Poisson
+    from cyclix_msparc_amd.system import make_system
+
+    sysm = make_system(args.workload)
+    g = sysm.grid
+    N, ns, npl = g.N, sysm.nstates, sysm.npl
+    dev = Device(0)
+    tdev = dev.torch_device
+    paths = [p for p in args.paths.split(",") if p]
+    unet = None
+    if "mixed" in paths and not args.no_learned:
+        from cyclix_msparc_amd.learned import UNet, default_weights_path
+        wp = default_weights_path(g)
+        if wp.exists():
+            unet = UNet(dev, g)
+            unet.load(wp)
+    results, states = {}, {}
+    for path in paths:
+        precision = "fp64" if path == "fp64" else "fp32"
+        results[path], states[path] = run_path(dev, sysm, args, precision, unet if path == "mixed" else None)
+
+    # the workload's Poisson solve on its own: rho = sum of atom-centred
     # Gaussians, cold start, 1e-8 relative residual (src/poissonSolve.m)
     pc = sysm.pseudocharge()
     b = torch.from_numpy(pc["b"]).to(tdev)
     rho_at = torch.from_numpy(pc["rho_at"]).to(tdev)
     poisson = Poisson(dev, g)
-    poisson.solve(rho_at, b, None, 1e-8, args.aar_maxit)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    _, info8 = poisson.solve(rho_at, b, None, 1e-8, args.aar_maxit)
-    torch.cuda.synchronize()
-    aar_ms = (time.perf_counter() - t) * 1e3
+    pois = {}
+    for mode in (["jacobi", "learned"] if unet is not None else ["jacobi"]):
+        if mode == "learned":
+            unet.attach(poisson)
+        poisson.solve(rho_at, b, None, 1e-8, 20)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, info8 = poisson.solve(rho_at, b, None, 1e-8, 5000)
+        torch.cuda.synchronize()
+        pois[mode] = {"ms": round((time.perf_counter() - t) * 1e3, 2), "iterations": int(info8.iterations),
+                      "converged": bool(info8.converged), "relres": float(info8.relres)}
+        if mode == "learned":
+            unet.detach(poisson)
 
     if args.calib_copy:
         src = torch.ones(1 << 26, dtype=torch.float64, device=tdev)
@@ -137,45 +197,42 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
 
-    a0, bup, cut = gs.bounds
-    aar_its = float(np.mean([r.aar_iterations for r in recs]))
+    head = "mixed" if "mixed" in results else paths[0]
+    H = results[head]
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sysm, sysm.projectors(), pc, gs.veff.cpu().numpy(),
-                           dict(cut=float(cut[0]), a0=float(a0[0]), bup=float(bup[0]), aar_its=aar_its),
-                           ms_step, args)
-
-    phases = {k: round(v[0] / args.steps, 3) for k, v in tm.items()}
+        st = states["fp64"] if "fp64" in states else states[head]
+        a0, bup, cut = st["bounds"]
+        cpu = cpu_baseline(sysm, sysm.projectors(), pc, st["veff"],
+                           dict(cut=float(cut[0]), a0=float(a0[0]), bup=float(bup[0]), aar_its=st["aar_its"]),
+                           H["ms_per_step"], args)
     out = {
         "metric": METRIC,
-        "value": round(ms_step, 3),
+        "value": H["ms_per_step"],
         "unit": "ms/SCF-step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": H["ms_per_step"],
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "complex128" if args.dtype == "c128" else "complex64",
+        "dtype": H["dtype"],
         "data": "synthetic (seeded psi0 ~ N(0,1); synthetic Gaussian pseudopotential)",
         "config": {"workload": f"{args.workload}: (10,0) CNT 1/5 sector, twist 0.05, 8 atoms, grid "
                                f"{g.nr}x{g.nt}x{g.nz} (N={N}), {ns} states, Chebyshev degree {npl}, "
-                               "1 Bloch channel; SCF iterations with Periodic-Pulay potential mixing, "
-                               f"SCF_tol {gs.opt.SCF_tol:.3e} (1e-5 Ha/atom), AAR Poisson tol "
-                               f"{gs.opt.poisson_tol:.3e}",
-                   "path": "FP64 GPU" if args.dtype == "c128" else "mixed (complex64 filter)"},
-        "roofline": {"bound": "hbm", "kernel": "stencil_step_kernel (fused H-apply + Chebyshev step)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4)},
-        "filter": {"ms": round(filt_ms_only, 3), "GB/s": round(filter_gbs, 1),
-                   "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4)},
-        "phase_ms_per_step": phases,
-        "scf": {"err": [round(r.err, 6) for r in recs], "Eatom": [round(r.Etotal / sysm.n_atm, 8) for r in recs],
-                "aar_iterations": [r.aar_iterations for r in recs]},
-        "poisson_1e-8": {"ms": round(aar_ms, 2), "iterations": int(info8.iterations),
-                         "converged": bool(info8.converged)},
+                               "1 Bloch channel; SCF iterations with Periodic-Pulay potential mixing at the "
+                               "SCF_tol of 1e-5 Ha/atom, AAR Poisson to 0.01 SCF_tol",
+                   "path": {"fp64": "FP64 GPU", "mixed": "mixed precision" + (" + learned Poisson correction"
+                                                                              if unet is not None else "")}[head]},
+        "roofline": H["roofline"],
+        "filter": H["filter"],
+        "phase_ms_per_step": H["phase_ms_per_step"],
+        "scf": H["scf"],
+        "paths": results,
+        "speedup_vs_fp64": (round(results["fp64"]["ms_per_step"] / H["ms_per_step"], 3)
+                            if "fp64" in results and head != "fp64" else None),
+        "poisson_1e-8": pois,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

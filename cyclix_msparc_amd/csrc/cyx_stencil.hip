@@ -16,6 +16,8 @@
 // compile-time constants (the plane loop is unrolled by 13).  An output plane
 // is complete FDN planes after it entered the queue; it then takes the KB
 // scatter (CSR by node), the recurrence, and is written in place over prev.
+#include <type_traits>
+
 #include "cyx_internal.hpp"
 
 namespace cyx {
@@ -31,6 +33,13 @@ constexpr int stencil_max_threads() {
 template <class T>
 constexpr int stencil_min_blocks() {
   return sizeof(T) >= 16 ? 1 : 2;
+}
+// z planes staged per barrier: complex128 (one workgroup per CU) takes two,
+// so that the LDS reads of one plane overlap the FMAs of the other and the
+// barriers halve; smaller types keep one and run two workgroups per CU
+template <class T>
+constexpr int stencil_pp() {
+  return sizeof(T) >= 16 ? 2 : 1;
 }
 
 template <class T>
@@ -90,9 +99,10 @@ __device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __device__ __forceinline__ void stencil_body(const StepParams<T>& P, int bid0, int nblk, T* lds) {
   using R = typename traits<T>::real;
+  constexpr int PP = stencil_pp<T>();
   const int TRH = P.trh;
   const int plane_sz = TRH * (P.nt + 2 * FDN);
-  T* Dsh = lds + 2 * plane_sz;  // per-column KB table D[il]
+  T* Dsh = lds + 2 * PP * plane_sz;  // per-column KB table D[il]
 
   int bid = bid0;
   if (P.xcd_remap) {
@@ -179,6 +189,7 @@ __device__ __forceinline__ void stencil_body(const StepParams<T>& P, int bid0, i
     ve = (rvalid && kk >= z0 && kk < z1) ? P.veff[(int64_t)kp * nrt + jr] : R(0);
   };
 
+  if constexpr (PP == 1) {
   // KB pipeline.  At the start of step kk: (hold, cur, nxt) are the CSR row
   // pointers of planes (kk, kk+1, kk+2) and echi/eil hold up to KB_SLOTS
   // entries of plane kk.  The step consumes plane kk's entries, issues the
@@ -325,11 +336,197 @@ __device__ __forceinline__ void stencil_body(const StepParams<T>& P, int bid0, i
       }
     }
   }
+  } else {
+
+  // ---- two planes per barrier (see stencil_pp)
+  // stage a plane: interior row, its r halo, and the theta halo copies with
+  // the Bloch phase
+  auto stage = [&](T* L, const T& xa, const T& xb) {
+    const int row = (j + FDN) * TRH;
+    L[row + i] = xa;
+    if (i < 2 * FDN) L[row + P.tr + i] = xb;
+    if (j < FDN) {
+      const int rw = (j + FDN + P.nt) * TRH;
+      L[rw + i] = cmul(P.ph_tr, xa);
+      if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tr, xb);
+    }
+    if (j >= P.nt - FDN) {
+      const int rw = (j + FDN - P.nt) * TRH;
+      L[rw + i] = cmul(P.ph_tl, xa);
+      if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tl, xb);
+    }
+  };
+  // KB pipeline, per plane slot p of a pair (planes t0+p): hold[p] is the
+  // node word of plane t0+p with its entries in echi2[p]/eil2[p]; nw[p] the
+  // word of plane t0+p+2.  Planes z0-6 .. z0-3 carry no entries, so the
+  // zero start is exact.
+  uint32_t holdA = 0u, holdB = 0u, nwA = 0u, nwB = 0u;
+  R echiA[KB_SLOTS], echiB[KB_SLOTS];
+  int eilA[KB_SLOTS], eilB[KB_SLOTS];
+#pragma unroll
+  for (int e = 0; e < KB_SLOTS; ++e) {
+    echiA[e] = echiB[e] = R(0);
+    eilA[e] = eilB[e] = 0;
+  }
+  // one plane's KB step: the scatter sum of plane kk (returned), then the
+  // entry loads of plane kk+2 and the node word of plane kk+4
+  auto kb_step = [&](int kk, uint32_t& hold, uint32_t& nw, R(&ech)[KB_SLOTS], int(&eix)[KB_SLOTS]) -> T {
+    T kb = zero<T>();
+    if (kk >= z0 && kk < z1 && rvalid) {
+      const int h0 = (int)(hold >> 6), cnt = (int)(hold & 63u);
+#pragma unroll
+      for (int e = 0; e < KB_SLOTS; ++e)
+        if (e < cnt) kb = rfma(ech[e], Dsh[eix[e]], kb);
+      for (int e = h0 + KB_SLOTS; e < h0 + cnt; ++e) kb = rfma(P.s_chi[e], Dsh[P.s_il[e]], kb);
+    }
+    hold = nw;
+    {
+      const int h0 = (int)(hold >> 6), cnt = (int)(hold & 63u);
+#pragma unroll
+      for (int e = 0; e < KB_SLOTS; ++e)
+        if (e < cnt) {
+          ech[e] = P.s_chi[h0 + e];
+          eix[e] = P.s_il[h0 + e];
+        }
+    }
+    nw = 0u;
+    if (rvalid && kk + 4 >= z0 && kk + 4 < z1) nw = P.s_pk[(int64_t)(kk + 4) * nrt + jr];
+    return kb;
+  };
+  // X_prev of the two planes emitted next (z0 and z0+1 first)
+  T pv2[2] = {zero<T>(), zero<T>()};
+  if constexpr (HAS_PREV) {
+    if (rvalid) {
+      const T* pp = P.prev + colbase + (int64_t)z0 * nrt + jr;
+      pv2[0] = P.nontemporal ? ntload(pp) : *pp;
+      if (z0 + 1 < z1) pv2[1] = P.nontemporal ? ntload(pp + nrt) : pp[nrt];
+    }
+  }
+  T xa2[2], xb2[2];
+  R ve2[2];
+  load_plane(kbeg, xa2[0], xb2[0], ve2[0]);
+  xa2[1] = xb2[1] = zero<T>();
+  ve2[1] = R(0);
+  if (nplanes > 1) load_plane(kbeg + 1, xa2[1], xb2[1], ve2[1]);
+  int buf = 0;
+  if constexpr (HAS_KB) __syncthreads();  // Dsh ready
+
+  for (int base = 0; base < nplanes; base += 2 * QN) {
+#pragma unroll
+    for (int v = 0; v < QN; ++v) {
+      const int t0 = base + 2 * v;
+      if (t0 < nplanes) {
+        const int kk0 = kbeg + t0;
+        T* Lp[2] = {lds + (2 * buf) * plane_sz, lds + (2 * buf + 1) * plane_sz};
+        R vc[2];
+        // the two planes of the pair, with the plane index a compile-time
+        // constant (the queue slots must be static register indices)
+        auto stage_plane = [&](auto pc) {
+          constexpr int p = decltype(pc)::value;
+          if (t0 + p < nplanes) {
+            stage(Lp[p], xa2[p], xb2[p]);
+            vc[p] = ve2[p];
+            if constexpr (HAS_KB) {
+              const T kb = p == 0 ? kb_step(kk0, holdA, nwA, echiA, eilA) : kb_step(kk0 + 1, holdB, nwB, echiB, eilB);
+              q[(2 * v + p) % QN] = add(q[(2 * v + p) % QN], kb);
+            }
+          }
+        };
+        stage_plane(std::integral_constant<int, 0>{});
+        stage_plane(std::integral_constant<int, 1>{});
+        __syncthreads();
+        // ---- prefetch the next pair
+        if (t0 + 2 < nplanes) load_plane(kk0 + 2, xa2[0], xb2[0], ve2[0]);
+        if (t0 + 3 < nplanes) load_plane(kk0 + 3, xa2[1], xb2[1], ve2[1]);
+        auto compute_plane = [&](auto pc) {
+          constexpr int p = decltype(pc)::value;
+          if (t0 + p < nplanes) {
+            const int u = 2 * v + p;
+            const int kk = kk0 + p;
+            const bool inner = kk >= z0 && kk < z1;
+            const T* L = Lp[p];
+            const int c = (j + FDN) * TRH + (i + FDN);
+            const T x0 = L[c];
+            T tt = zero<T>();
+            T st = zero<T>();
+#pragma unroll
+            for (int d = 1; d <= FDN; ++d) {
+              const T xp = L[c + d * TRH];
+              const T xm = L[c - d * TRH];
+              st = rfma(P.w2[d], add(xp, xm), st);
+              if constexpr (HELICAL) tt = rfma(P.w1[d], sub(xp, xm), tt);
+            }
+            if (inner) {
+              T sr = zero<T>();
+              T dr = zero<T>();
+#pragma unroll
+              for (int d = 1; d <= FDN; ++d) {
+                const T xp = L[c + d];
+                const T xm = L[c - d];
+                sr = rfma(P.w2[d], add(xp, xm), sr);
+                dr = rfma(P.w1[d], sub(xp, xm), dr);
+              }
+              T self = rfma(P.s_r2, sr, rmul(cdg + vc[p], x0));
+              self = rfma(rfac, dr, self);
+              self = rfma(cth, st, self);
+              q[u % QN] = add(q[u % QN], self);
+            }
+            {
+              const T xz = rmul(P.s_z2, x0);
+              T tz = zero<T>();
+              if constexpr (HELICAL) tz = rmul(P.s_x, tt);
+#pragma unroll
+              for (int d = 1; d <= FDN; ++d) {
+                T& qa = q[(u - d + QN) % QN];
+                T& qb = q[(u + d) % QN];
+                qa = rfma(P.w2[d], xz, qa);
+                qb = rfma(P.w2[d], xz, qb);
+                if constexpr (HELICAL) {
+                  qa = rfma(P.w1[d], tz, qa);
+                  qb = rfma(-P.w1[d], tz, qb);
+                }
+              }
+            }
+            // output plane kk-FDN is complete; emit it before the next plane
+            // of the pair reuses its queue slot
+            const int ko = kk - FDN;
+            T& qo = q[(u + QN - FDN) % QN];
+            if (ko >= z0 && ko < z1 && rvalid) {
+              T y = rmul(P.alpha, qo);
+              if constexpr (HAS_PREV) y = rfma(-P.beta, pv2[p], y);
+              T* po = P.out + colbase + (int64_t)ko * nrt + jr;
+              if (P.nontemporal)
+                ntstore(po, y);
+              else
+                *po = y;
+              if constexpr (HAS_PREV) {
+                if (ko + 2 < z1) {
+                  const T* pp = P.prev + colbase + (int64_t)(ko + 2) * nrt + jr;
+                  pv2[p] = P.nontemporal ? ntload(pp) : *pp;
+                }
+              }
+            }
+            qo = zero<T>();
+          }
+        };
+        compute_plane(std::integral_constant<int, 0>{});
+        compute_plane(std::integral_constant<int, 1>{});
+        buf ^= 1;
+      }
+    }
+  }
+  }
+}
+
+// waves per SIMD: 2 for complex128, 4 (two 8-wave workgroups per CU) otherwise
+template <class T>
+constexpr int stencil_waves_per_eu() {
+  return 2 * stencil_min_blocks<T>();
 }
 
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
-__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
-    stencil_step_kernel(const StepParams<T> P) {
+__global__ void __launch_bounds__(stencil_max_threads<T>())
+    __attribute__((amdgpu_waves_per_eu(stencil_waves_per_eu<T>()))) stencil_step_kernel(const StepParams<T> P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   stencil_body<T, HELICAL, HAS_PREV, HAS_KB>(P, blockIdx.x, gridDim.x, reinterpret_cast<T*>(smem_raw));
 }
@@ -344,7 +541,8 @@ struct BatchEntry {
 };
 
 template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
-__global__ void __launch_bounds__(stencil_max_threads<T>(), stencil_min_blocks<T>())
+__global__ void __launch_bounds__(stencil_max_threads<T>())
+    __attribute__((amdgpu_waves_per_eu(stencil_waves_per_eu<T>())))
     stencil_batch_kernel(const BatchEntry<T>* __restrict__ E, int nch) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   int c = 0;
@@ -529,14 +727,21 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   Lc.P = P;
   Lc.nblk = nblk;
   Lc.nthr = P.tr * g.nt;
-  Lc.shm = (2 * (size_t)P.trh * (g.nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T);
+  Lc.shm = (2 * stencil_pp<T>() * (size_t)P.trh * (g.nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T);
   Lc.hel = g.cell_typ == 4 || g.cell_typ == 5;
   Lc.kb = K.nproj > 0;
   return CYX_OK;
 }
 
+// dynamic LDS above 64 KB (two-plane staging of complex128) must be allowed
+// per kernel; gfx950 has 160 KB per CU
+static void allow_lds(const void* fn, size_t shm) {
+  if (shm > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+}
+
 template <class T, bool HEL, bool PREV, bool KB>
 static void launch_variant(const StepParams<T>& P, int nblk, int nthr, size_t shm, hipStream_t st) {
+  allow_lds((const void*)&stencil_step_kernel<T, HEL, PREV, KB>, shm);
   hipLaunchKernelGGL((stencil_step_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
 }
 
@@ -607,6 +812,7 @@ int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void
 // ---------------------------------------------------------------- batched filter plans
 template <class T, bool HEL, bool PREV, bool KB>
 static void launch_batch_variant(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, hipStream_t st) {
+  allow_lds((const void*)&stencil_batch_kernel<T, HEL, PREV, KB>, shm);
   hipLaunchKernelGGL((stencil_batch_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, E, nch);
 }
 
@@ -721,7 +927,7 @@ int plan_setup(cyx_filter_plan_s* pl) {
     if (pl->kb)
       CYX_CHECK(ensure(h->kbC, (size_t)pl->ncol[c] * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
     const size_t trh = (size_t)pl->tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
-    shm = std::max(shm, (2 * trh * (nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T));
+    shm = std::max(shm, (2 * stencil_pp<T>() * trh * (nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T));
     for (int v = 0; v < 2; ++v) {
       GatherEntry<T>& g = G[(size_t)v * nch + c];
       g.Y = (const T*)(v == 0 ? pl->A[c] : pl->B[c]);
