@@ -8,9 +8,11 @@ eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24.
 
 Two paths are timed: the FP64 path (complex128 throughout, Jacobi-AAR) and
 the mixed path (complex64 psi, filter and H-applies with FP64 accumulation of
-the Gram products, FP64 dense eigensolve, density and energies; the learned
-Poisson correction when its weights for this grid are present).  `value` is
-the mixed path's time per step; `speedup_vs_fp64` compares the two.
+the Gram products, FP64 dense eigensolve, density and energies; with
+--learned also the learned Poisson correction).  `value` is the mixed path's
+time per step; `speedup_vs_fp64` compares the two.  The standalone Poisson
+solve is timed with Jacobi-AAR and, when trained weights for the grid exist,
+with the learned correction.
 
 One step = one SCF iteration of src/scf.m on the device, at the SCF_tol of
 a 1e-5 Ha/atom energy target (src/initialization.m:516-552): CheFSI pass
@@ -40,6 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per recurrence launch of the fused stencil at this workload, from
+# the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
+# calibration) recorded in profiles/scf_step_cnt10_helical.md
+MEASURED_TRAFFIC = {("cnt10_helical", 16): 4.2047e9, ("cnt10_helical", 8): 2.1830e9}
 METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM peak"
 
 
@@ -52,7 +58,11 @@ def parse():
     p.add_argument("--paths", default="fp64,mixed",
                    help="comma list of fp64 (complex128, Jacobi-AAR) and mixed (complex64 filter and subspace "
                         "inputs with FP64 accumulation, learned Poisson correction when weights exist)")
-    p.add_argument("--no-learned", action="store_true")
+    p.add_argument("--learned", action="store_true",
+                   help="use the learned Poisson correction inside the mixed path's SCF steps (it is measured "
+                        "slower than Jacobi-AAR, see poisson_1e-8); the standalone solve compares both whenever "
+                        "weights exist")
+    p.add_argument("--no-learned", action="store_true", help="skip everything learned (no weights loaded)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
@@ -125,7 +135,8 @@ def run_path(dev, sysm, args, precision, unet=None):
         "roofline": {"bound": "hbm", "kernel": f"stencil_step_kernel<{'c128' if precision == 'fp64' else 'c64'}> "
                                                "(fused H-apply + Chebyshev step)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, bsz)),
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4)},
         "filter": {"ms": round(filt_ms, 3), "GB/s": round(filter_gbs, 1),
                    "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4)},
@@ -169,7 +180,8 @@ def main():
     results, states = {}, {}
     for path in paths:
         precision = "fp64" if path == "fp64" else "fp32"
-        results[path], states[path] = run_path(dev, sysm, args, precision, unet if path == "mixed" else None)
+        results[path], states[path] = run_path(dev, sysm, args, precision,
+                                               unet if (path == "mixed" and args.learned) else None)
 
     # the workload's Poisson solve on its own: rho = sum of atom-centred
     # Gaussians, cold start, 1e-8 relative residual (src/poissonSolve.m)
@@ -224,7 +236,8 @@ def main():
                                "1 Bloch channel; SCF iterations with Periodic-Pulay potential mixing at the "
                                "SCF_tol of 1e-5 Ha/atom, AAR Poisson to 0.01 SCF_tol",
                    "path": {"fp64": "FP64 GPU", "mixed": "mixed precision" + (" + learned Poisson correction"
-                                                                              if unet is not None else "")}[head]},
+                                                                              if args.learned and unet is not None
+                                                                              else "")}[head]},
         "roofline": H["roofline"],
         "filter": H["filter"],
         "phase_ms_per_step": H["phase_ms_per_step"],

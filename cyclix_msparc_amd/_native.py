@@ -60,7 +60,7 @@ class Tuning(C.Structure):
 
 class AarInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("relres", C.c_double),
-                ("precond", C.c_int32), ("reserved", C.c_int32)]
+                ("precond", C.c_int32), ("pinv_eig", C.c_int32)]
 
 
 class SternInfo(C.Structure):

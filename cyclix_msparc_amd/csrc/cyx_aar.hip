@@ -3,18 +3,23 @@
 // extras/AAR_old.m:45-46,73-75).  Operator: S.Lap_std at k = 0
 // (src/initialization.m:734-741), applied matrix-free.
 //
-// One AAR iteration is three launches (four on Anderson steps):
-//   1. theta_kernel: T = (-2 tau D_theta) x  (helical cross term, separable)
-//   2. step kernel:  res1 = f - Lap x, res = res1 ./ diag, history columns,
-//      on Richardson steps x_new = x + omega res (written speculatively into
-//      the other x buffer), per-block partials of ||res1||^2 and, on
-//      Anderson steps, of DF'DF and DF'res;
-//   3. finalize (one block): fixed-order sums, convergence flag, and on
-//      Anderson steps gamma = pinv(DF'DF) DF'res;
-//   4. (Anderson) x_new = x + beta res - (DX + beta DF) gamma.
-// Kernels of an iteration that starts after convergence return at once, so
-// the host enqueues iterations in chunks and polls a device flag between
-// chunks.  Every reduction has a fixed order: a solve is bitwise reproducible.
+// The 2.2 MB-per-vector problem is latency-bound, so an iteration is kept to
+// as few dependent launches as possible:
+//   Richardson step (two launches):
+//     1. theta_kernel: T = (-2 tau D_theta) x (helical cross term), plus one
+//        extra block that finalises the previous Richardson iteration: the
+//        fixed-order sum of its partials, relres and the convergence flag;
+//     2. step kernel: res1 = f - Lap x, res = res1 ./ diag, history columns,
+//        x_new = x + omega res (written speculatively into the other x
+//        buffer), per-block partials of ||res1||^2.
+//   Anderson step (four launches): theta (+ finalise of the previous
+//   iteration), step kernel with the DF'DF and DF'res partials, a one-block
+//   finalize that also computes gamma = pinv(DF'DF) DF'res on one wavefront,
+//   and x_new = x + beta res - (DX + beta DF) gamma.
+// Partials are stored column-major, part[d][block].  Step kernels of an
+// iteration after convergence store nothing, so the host enqueues iterations
+// in chunks and polls a device flag between chunks.  Every reduction has a
+// fixed order: a solve is bitwise reproducible.
 #include "cyx_internal.hpp"
 #include "cyx_reduce.hpp"
 
@@ -24,17 +29,16 @@ struct AarState {
   int done;       // 1 converged, 2 hit max_iter
   int iters;      // count at exit (src/aar.m:394-397 reports count-1 after the loop)
   int final_buf;  // which x buffer holds the returned iterate
-  int counter;    // last-block-done ticket
+  int pinv_eig;   // Anderson steps that took the eigen-decomposition pseudo-inverse
   double relres;  // relres of the returned iterate
   double nb_inv;  // 1 / ||f||
   double gamma[16];
 };
 
-
 struct PoissonOp {
-  int nr, nt, nz;
+  int nr, nt, nz, N;
   int helical;
-  const double* coef;   // [nr][RCOEF]: c+[6], c-[6], c_theta, diag (Lap_std, no -1/2)
+  const double* coefT;  // [RCOEF][nr]: c+[6], c-[6], c_theta, diag (Lap_std, no -1/2), radius fastest
   double w2[FDN + 1];   // w2_p (theta second derivative, times c_theta per radius)
   double wz2[FDN + 1];  // w2_q / dz^2
   double wt1[FDN + 1];  // -2 tau w1_p / dt
@@ -43,30 +47,64 @@ struct PoissonOp {
 
 __device__ inline int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Launch shape of the per-point AAR kernels: 64 radial x 4 theta threads per
-// block, grid (ceil(nr/64), ceil(nt/4), nz); 32-bit index arithmetic.
-constexpr int TX = 64, TY = 4;
+// Per-point kernels: 1-D blocks of PB threads, one grid point per thread in
+// the r-fastest linear order (N < 2^31 is checked on the host).
+constexpr int PB = 512;
 
 struct Pt {
-  int i, j, k;
+  int i, j, k, n;
   bool ok;
-  int n;
 };
-__device__ inline Pt point_of(const int nr, const int nt) {
-  Pt p;
-  p.i = blockIdx.x * TX + threadIdx.x;
-  p.j = blockIdx.y * TY + threadIdx.y;
-  p.k = blockIdx.z;
-  p.ok = p.i < nr && p.j < nt;
-  p.n = (p.k * nt + p.j) * nr + p.i;
-  return p;
+__device__ inline Pt point_of(const PoissonOp& P) {
+  Pt q;
+  q.n = blockIdx.x * PB + threadIdx.x;
+  q.ok = q.n < P.N;
+  const int nrt = P.nr * P.nt;
+  q.k = q.n / nrt;
+  const int rem = q.n - q.k * nrt;
+  q.j = rem / P.nr;
+  q.i = rem - q.j * P.nr;
+  return q;
 }
 
-// T[n] = sum_p wt1_p (x(j+p) - x(j-p))
-__global__ void __launch_bounds__(AB) theta_kernel(PoissonOp P, const double* __restrict__ x, double* __restrict__ T,
-                                                   const AarState* S) {
-  if (S->done) return;
-  const Pt q = point_of(P.nr, P.nt);
+__device__ inline double diag_at(const PoissonOp& P, int i) { return P.coefT[(2 * FDN + 1) * P.nr + i]; }
+
+// Finalise of a Richardson iteration (src/aar.m:43-45): relres of x_prev
+// from the nblk partials of ||res1||^2, convergence and iteration-cap flags.
+struct FinArgs {
+  const double* part;
+  int nblk, count, max_iter, xbuf;
+  double tol;
+  AarState* S;
+};
+
+__device__ void finalize_richardson(const FinArgs& F) {
+  if (F.S->done) return;
+  __shared__ double tot[1];
+  column_sums(F.part, F.nblk, 1, tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double relres = sqrt(tot[0]) * F.S->nb_inv;
+    F.S->relres = relres;
+    F.S->iters = F.count;
+    F.S->final_buf = F.xbuf;
+    if (relres <= F.tol)
+      F.S->done = 1;
+    else if (F.count >= F.max_iter)
+      F.S->done = 2;
+  }
+}
+
+// Blocks [0, ntb): T[n] = sum_p wt1_p (x(j+p) - x(j-p)).  Block ntb, when
+// launched, finalises the previous Richardson iteration.  T is scratch, so
+// the theta blocks run whatever the convergence state.
+__global__ void __launch_bounds__(PB) theta_kernel(PoissonOp P, const double* __restrict__ x, double* __restrict__ T,
+                                                   int ntb, FinArgs F) {
+  if ((int)blockIdx.x == ntb) {
+    finalize_richardson(F);
+    return;
+  }
+  const Pt q = point_of(P);
   if (!q.ok) return;
   const int base = q.n - q.j * P.nr;  // (i, 0, k)
   double t = 0;
@@ -76,29 +114,29 @@ __global__ void __launch_bounds__(AB) theta_kernel(PoissonOp P, const double* __
   T[q.n] = t;
 }
 
-__device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x, const double* __restrict__ T, int i,
-                                int j, int k, int n) {
-  const int nrt = P.nr * P.nt;
-  const double* cf = P.coef + i * RCOEF;
-  double acc = cf[2 * FDN + 1] * x[n];
+__device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x, const double* __restrict__ T,
+                                const Pt& q) {
+  const int nr = P.nr, nrt = P.nr * P.nt, i = q.i, j = q.j, k = q.k, n = q.n;
+  const double* cf = P.coefT + i;
+  double acc = cf[(2 * FDN + 1) * nr] * x[n];
 #pragma unroll
   for (int p = 1; p <= FDN; ++p) {
-    if (i + p < P.nr) acc = __builtin_fma(cf[p - 1], x[n + p], acc);
-    if (i - p >= 0) acc = __builtin_fma(cf[FDN + p - 1], x[n - p], acc);
+    if (i + p < nr) acc = __builtin_fma(cf[(p - 1) * nr], x[n + p], acc);
+    if (i - p >= 0) acc = __builtin_fma(cf[(FDN + p - 1) * nr], x[n - p], acc);
   }
   const int rowk = k * nrt + i;
   double st = 0.0;
 #pragma unroll
   for (int p = 1; p <= FDN; ++p)
-    st = __builtin_fma(P.w2[p], x[rowk + wrapi(j + p, P.nt) * P.nr] + x[rowk + wrapi(j - p, P.nt) * P.nr], st);
-  acc = __builtin_fma(cf[2 * FDN], st, acc);
-  const int colj = j * P.nr + i;
+    st = __builtin_fma(P.w2[p], x[rowk + wrapi(j + p, P.nt) * nr] + x[rowk + wrapi(j - p, P.nt) * nr], st);
+  acc = __builtin_fma(cf[2 * FDN * nr], st, acc);
+  const int colj = j * nr + i;
 #pragma unroll
-  for (int q = 1; q <= FDN; ++q) {
-    const int kp = wrapi(k + q, P.nz) * nrt + colj;
-    const int km = wrapi(k - q, P.nz) * nrt + colj;
-    acc = __builtin_fma(P.wz2[q], x[kp] + x[km], acc);
-    if (P.helical) acc = __builtin_fma(P.wz1[q], T[kp] - T[km], acc);
+  for (int qq = 1; qq <= FDN; ++qq) {
+    const int kp = wrapi(k + qq, P.nz) * nrt + colj;
+    const int km = wrapi(k - qq, P.nz) * nrt + colj;
+    acc = __builtin_fma(P.wz2[qq], x[kp] + x[km], acc);
+    if (P.helical) acc = __builtin_fma(P.wz1[qq], T[kp] - T[km], acc);
   }
   return acc;
 }
@@ -110,55 +148,52 @@ __device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x
 // residual norm whatever the network returns.
 
 // r1 = f - Lap x
-__global__ void __launch_bounds__(AB) resid_kernel(PoissonOp P, const double* __restrict__ f,
+__global__ void __launch_bounds__(PB) resid_kernel(PoissonOp P, const double* __restrict__ f,
                                                    const double* __restrict__ x, const double* __restrict__ T,
                                                    double* __restrict__ r1, const AarState* S) {
   if (S->done) return;
-  const Pt q = point_of(P.nr, P.nt);
+  const Pt q = point_of(P);
   if (!q.ok) return;
-  r1[q.n] = f[q.n] - lap_at(P, x, T, q.i, q.j, q.k, q.n);
+  r1[q.n] = f[q.n] - lap_at(P, x, T, q);
 }
 
-__global__ void __launch_bounds__(AB) ls_dir_kernel(PoissonOp P, const double* __restrict__ r1,
+__global__ void __launch_bounds__(PB) ls_dir_kernel(PoissonOp P, const double* __restrict__ r1,
                                                     const float* __restrict__ u, double* __restrict__ d,
                                                     const AarState* S) {
   if (S->done) return;
-  const Pt q = point_of(P.nr, P.nt);
+  const Pt q = point_of(P);
   if (!q.ok) return;
-  d[q.n] = r1[q.n] / P.coef[q.i * RCOEF + 2 * FDN + 1] + (double)u[q.n];
+  d[q.n] = r1[q.n] / diag_at(P, q.i) + (double)u[q.n];
 }
 
-__global__ void __launch_bounds__(AB) ls_apply_kernel(PoissonOp P, const double* __restrict__ d,
+__global__ void __launch_bounds__(PB) ls_apply_kernel(PoissonOp P, const double* __restrict__ d,
                                                       const double* __restrict__ T, const double* __restrict__ r1,
                                                       double* __restrict__ part, const AarState* S) {
   if (S->done) return;
-  __shared__ double red[AB];
-  const Pt q = point_of(P.nr, P.nt);
-  double a = 0, b = 0;
+  __shared__ double red[(PB / 64) * 2];
+  const Pt q = point_of(P);
+  double v[2] = {0, 0};
   if (q.ok) {
-    const double w = lap_at(P, d, T, q.i, q.j, q.k, q.n);
-    a = w * r1[q.n];
-    b = w * w;
+    const double w = lap_at(P, d, T, q);
+    v[0] = w * r1[q.n];
+    v[1] = w * w;
   }
-  const double sa = block_sum(a, red);
-  const double sb = block_sum(b, red);
-  if (tlin() == 0) {
-    part[(int64_t)blin() * 2] = sa;
-    part[(int64_t)blin() * 2 + 1] = sb;
-  }
+  const double s = block_sums<2, PB>(v, red);
+  if (threadIdx.x < 2) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
 }
 
 __global__ void __launch_bounds__(AB) ls_finalize_kernel(const double* part, int nblk, AarState* S) {
   if (S->done) return;
   __shared__ double tot[2];
-  reduce_partials(part, nblk, 2, tot);
+  column_sums(part, nblk, 2, tot);
+  __syncthreads();
   if (threadIdx.x == 0) S->gamma[15] = tot[1] > 0 ? tot[0] / tot[1] : 0.0;
 }
 
-__global__ void __launch_bounds__(AB) ls_axpy_kernel(PoissonOp P, double* __restrict__ x,
+__global__ void __launch_bounds__(PB) ls_axpy_kernel(PoissonOp P, double* __restrict__ x,
                                                      const double* __restrict__ d, const AarState* S) {
   if (S->done) return;
-  const Pt q = point_of(P.nr, P.nt);
+  const Pt q = point_of(P);
   if (!q.ok) return;
   x[q.n] += S->gamma[15] * d[q.n];
 }
@@ -173,7 +208,8 @@ __global__ void __launch_bounds__(AB) norm_kernel(const double* __restrict__ f, 
 
 __global__ void __launch_bounds__(AB) norm_finalize_kernel(const double* part, int nblk, AarState* S) {
   __shared__ double tot[1];
-  reduce_partials(part, nblk, 1, tot);
+  column_sums(part, nblk, 1, tot);
+  __syncthreads();
   if (threadIdx.x == 0) S->nb_inv = 1.0 / sqrt(tot[0]);
 }
 
@@ -186,7 +222,7 @@ struct StepArgs {
   const double* fold;   // previous preconditioned residual
   double* DX;
   double* DF;
-  double* part;
+  double* part;         // [nd][gridDim.x]
   AarState* S;
   int hcol;             // history column written this iteration (-1: none)
   int anderson;         // 1 on Anderson steps
@@ -194,68 +230,73 @@ struct StepArgs {
   double tol, omega;
 };
 
-// MH: compile-time history size for the Anderson dot products (0: none)
+// MH: compile-time history size of the Anderson dot products (0: Richardson
+// step, which writes x_new; MH > 0: Anderson step, which writes the partials
+// of DF'DF and DF'res instead)
 template <int MH>
-__global__ void __launch_bounds__(AB) aar_step_kernel(PoissonOp P, StepArgs A) {
-  if (A.S->done) return;
-  __shared__ double red[AB];
+__global__ void __launch_bounds__(PB) aar_step_kernel(PoissonOp P, StepArgs A) {
   constexpr int ND = MH * (MH + 1) / 2 + MH;
-  const int64_t N = (int64_t)P.nr * P.nt * P.nz;
-  double nrm = 0;
-  double acc[ND > 0 ? ND : 1];
+  __shared__ double red[(PB / 64) * (1 + ND)];
+  const int done = A.S->done;  // stores are skipped after convergence; the loads need not wait for it
+  const int64_t N = P.N;
+  double v[1 + ND];
 #pragma unroll
-  for (int d = 0; d < (ND > 0 ? ND : 1); ++d) acc[d] = 0;
-  const Pt q = point_of(P.nr, P.nt);
+  for (int d = 0; d < 1 + ND; ++d) v[d] = 0;
+  const Pt q = point_of(P);
+  double x = 0, r = 0, xold = 0, rold = 0;
   if (q.ok) {
     const int n = q.n;
-    const double x = A.xp[n];
-    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q.i, q.j, q.k, n);
-    nrm = r1 * r1;
-    const double r = r1 / P.coef[q.i * RCOEF + 2 * FDN + 1];
+    x = A.xp[n];
+    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q);
+    v[0] = r1 * r1;
+    r = r1 / diag_at(P, q.i);
+    if (A.hcol >= 0) {
+      xold = A.xo[n];
+      rold = A.fold[n];
+    }
+  }
+  if (done) return;
+  if (q.ok) {
+    const int n = q.n;
     A.res[n] = r;
     if (A.hcol >= 0) {
-      A.DX[(int64_t)A.hcol * N + n] = x - A.xo[n];
-      A.DF[(int64_t)A.hcol * N + n] = r - A.fold[n];
+      A.DX[(int64_t)A.hcol * N + n] = x - xold;
+      A.DF[(int64_t)A.hcol * N + n] = r - rold;
     }
-    if (!A.anderson) {
+    if constexpr (MH == 0) {
       A.xo[n] = __builtin_fma(A.omega, r, x);
-    } else if constexpr (MH > 0) {
+    } else {
       double df[MH];
 #pragma unroll
-      for (int a = 0; a < MH; ++a) df[a] = A.DF[(int64_t)a * N + n];
-      int d = 0;
+      for (int a = 0; a < MH; ++a) df[a] = a == A.hcol ? r - rold : A.DF[(int64_t)a * N + n];
+      int d = 1;
 #pragma unroll
       for (int a = 0; a < MH; ++a)
 #pragma unroll
-        for (int b = a; b < MH; ++b) acc[d++] = df[a] * df[b];
+        for (int b = a; b < MH; ++b) v[d++] = df[a] * df[b];
 #pragma unroll
-      for (int a = 0; a < MH; ++a) acc[d++] = df[a] * r;
+      for (int a = 0; a < MH; ++a) v[d++] = df[a] * r;
     }
   }
-  const int nd = 1 + (A.anderson ? ND : 0);
-  {
-    const double s = block_sum(nrm, red);
-    if (tlin() == 0) A.part[(int64_t)blin() * nd] = s;
-  }
-  if (A.anderson) {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const double s = block_sum(acc[d], red);
-      if (tlin() == 0) A.part[(int64_t)blin() * nd + 1 + d] = s;
-    }
-  }
+  const double s = block_sums<1 + ND, PB>(v, red);
+  if ((int)threadIdx.x < 1 + ND) A.part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
 }
 
-// one block: fixed-order sum of the step's partials, convergence decision
-// (src/aar.m:43-45) and, on Anderson steps, gamma = pinv(DF'DF) DF'res
+// one block: fixed-order sums of an Anderson step's partials, convergence
+// decision (src/aar.m:43-45) and, unless converged, gamma = pinv(DF'DF)
+// DF'res (src/aar.m:59) on wavefront 0: by Cholesky when DF'DF is certified
+// full rank under pinv's tolerance (then pinv = inverse), else by the
+// eigen-decomposition pseudo-inverse
 template <int MH>
-__global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, int nblk, int anderson, int count,
-                                                          int max_iter, int xbuf, double tol, AarState* S) {
+__global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, int nblk, int count, int max_iter,
+                                                          int xbuf, double tol, AarState* S) {
   if (S->done) return;
   constexpr int ND = MH * (MH + 1) / 2 + MH;
   __shared__ double tot[ND + 1];
-  const int nd = 1 + (anderson ? ND : 0);
-  reduce_partials(part, nblk, nd, tot);
+  __shared__ PinvShared<MH> W;
+  __shared__ int go;
+  column_sums(part, nblk, 1 + ND, tot);
+  __syncthreads();
   if (threadIdx.x == 0) {
     const double relres = sqrt(tot[0]) * S->nb_inv;
     S->relres = relres;
@@ -265,38 +306,36 @@ __global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, in
       S->done = 1;
     else if (count >= max_iter)
       S->done = 2;
-    if constexpr (MH > 0) {
-      if (anderson && !S->done) {
-        double Am[MH * MH], bb[MH], g[MH];
-        int d = 1;
-#pragma unroll
-        for (int a = 0; a < MH; ++a)
-#pragma unroll
-          for (int b = a; b < MH; ++b) {
-            Am[a * MH + b] = tot[d];
-            Am[b * MH + a] = tot[d];
-            ++d;
-          }
-#pragma unroll
-        for (int a = 0; a < MH; ++a) bb[a] = tot[d++];
-        pinv_solve<MH>(Am, bb, g);
-#pragma unroll
-        for (int a = 0; a < MH; ++a) S->gamma[a] = g[a];
-      }
-    }
+    go = S->done ? 0 : 1;
   }
+  __syncthreads();
+  if (!go || threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < MH * MH; e += 64) {
+    const int a = e / MH, b = e % MH, lo = a < b ? a : b, hi = a < b ? b : a;
+    // packed upper-triangle index of (lo, hi): rows lo' < lo hold MH - lo' entries
+    W.A[e] = tot[1 + lo * MH - lo * (lo - 1) / 2 + (hi - lo)];
+  }
+  if (lane < MH) W.b[lane] = tot[1 + MH * (MH + 1) / 2 + lane];
+  wave_sync();
+  if (!chol_solve_wave<MH>(W, lane)) {
+    pinv_solve_wave<MH>(W, lane);
+    if (lane == 0) S->pinv_eig += 1;
+  }
+  if (lane < MH) S->gamma[lane] = W.out[lane];
 }
 
 // x_new = x_prev + beta res - (DX + beta DF) gamma   (src/aar.m:59-60)
 template <int MH>
-__global__ void __launch_bounds__(AB) anderson_update_kernel(int nr, int nt, int64_t N, const double* __restrict__ xp,
+__global__ void __launch_bounds__(PB) anderson_update_kernel(PoissonOp P, const double* __restrict__ xp,
                                                              const double* __restrict__ res, double* __restrict__ xnew,
                                                              const double* __restrict__ DX,
                                                              const double* __restrict__ DF, double beta,
                                                              const AarState* S) {
   if (S->done) return;
-  const Pt q = point_of(nr, nt);
+  const Pt q = point_of(P);
   if (!q.ok) return;
+  const int64_t N = P.N;
   const int n = q.n;
   double corr = 0;
 #pragma unroll
@@ -323,13 +362,13 @@ int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f
 }
 
 template <int MH>
-static void launch_anderson(dim3 grid, int nblk, hipStream_t st, const PoissonOp& P, const StepArgs& A, int64_t N,
-                            double beta, double* xnew) {
-  hipLaunchKernelGGL(aar_step_kernel<MH>, grid, dim3(TX, TY), 0, st, P, A);
-  hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nblk, 1, A.count,
+static void launch_anderson(int nblk, hipStream_t st, const PoissonOp& P, const StepArgs& A, double beta,
+                            double* xnew) {
+  hipLaunchKernelGGL(aar_step_kernel<MH>, dim3(nblk), dim3(PB), 0, st, P, A);
+  hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nblk, A.count,
                      A.max_iter, A.xbuf, A.tol, A.S);
-  hipLaunchKernelGGL(anderson_update_kernel<MH>, grid, dim3(TX, TY), 0, st, P.nr, P.nt, N, A.xp, (const double*)A.res,
-                     xnew, (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
+  hipLaunchKernelGGL(anderson_update_kernel<MH>, dim3(nblk), dim3(PB), 0, st, P, A.xp, (const double*)A.res, xnew,
+                     (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
 }
 
 static PoissonOp make_op(const cyx_poisson_s* ps) {
@@ -337,7 +376,8 @@ static PoissonOp make_op(const cyx_poisson_s* ps) {
   P.nr = ps->g.nr;
   P.nt = ps->g.nt;
   P.nz = ps->g.nz;
-  P.coef = ps->coef;
+  P.N = (int)ps->N();
+  P.coefT = ps->coefT;
   P.helical = (ps->g.cell_typ == 4 || ps->g.cell_typ == 5) ? 1 : 0;
   {
     double fact[2 * FDN + 2], w2[FDN + 1], w1[FDN + 1];
@@ -361,24 +401,22 @@ static PoissonOp make_op(const cyx_poisson_s* ps) {
 }
 
 // y = Lap_std x
-__global__ void __launch_bounds__(AB) apply_kernel(PoissonOp P, const double* __restrict__ x,
+__global__ void __launch_bounds__(PB) apply_kernel(PoissonOp P, const double* __restrict__ x,
                                                    const double* __restrict__ T, double* __restrict__ y) {
-  const Pt q = point_of(P.nr, P.nt);
+  const Pt q = point_of(P);
   if (!q.ok) return;
-  y[q.n] = lap_at(P, x, T, q.i, q.j, q.k, q.n);
+  y[q.n] = lap_at(P, x, T, q);
 }
 
 int poisson_apply(cyx_poisson_s* ps, const double* x, double* y) {
   const int64_t N = ps->N();
   hipStream_t st = ps->ctx->stream;
   const PoissonOp P = make_op(ps);
-  const dim3 grid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
-  CYX_CHECK(ensure(ps->tbuf, (size_t)N * sizeof(double) + sizeof(AarState)) == CYX_OK, "allocation failed");
+  const int nblk = (int)ceil_div(N, PB);
+  CYX_CHECK(ensure(ps->tbuf, (size_t)N * sizeof(double)) == CYX_OK, "allocation failed");
   double* T = (double*)ps->tbuf.p;
-  AarState* S = (AarState*)(T + N);
-  CYX_HIP(hipMemsetAsync(S, 0, sizeof(AarState), st));
-  if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, x, T, (const AarState*)S);
-  hipLaunchKernelGGL(apply_kernel, grid, dim3(TX, TY), 0, st, P, x, (const double*)T, y);
+  if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, x, T, nblk, FinArgs{});
+  hipLaunchKernelGGL(apply_kernel, dim3(nblk), dim3(PB), 0, st, P, x, (const double*)T, y);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
@@ -389,14 +427,15 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   CYX_CHECK(m >= 1 && m <= 8, "AAR history m must be in [1, 8]");
   CYX_CHECK(p >= 1 && max_iter >= 1 && tol > 0, "invalid AAR parameters");
   const int64_t N = ps->N();
+  CYX_CHECK(N < ((int64_t)1 << 31) - PB, "grid too large for 32-bit point indices");
   hipStream_t st = ps->ctx->stream;
-  const dim3 pgrid(ceil_div(ps->g.nr, TX), ceil_div(ps->g.nt, TY), ps->g.nz);
-  const int nblk = (int)(pgrid.x * pgrid.y * pgrid.z);
+  const int nblk = (int)ceil_div(N, PB);
   const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
   const int ndmax = 1 + m * (m + 1) / 2 + m;
   // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
   const size_t vec = (size_t)N * sizeof(double);
-  const size_t state_end = vec * (4 + 2 * (size_t)m) + (size_t)nblk * ndmax * sizeof(double) + sizeof(AarState) + 256;
+  const size_t state_end = vec * (4 + 2 * (size_t)m) +
+                           (size_t)std::max(nblk, nblk_norm) * ndmax * sizeof(double) + sizeof(AarState) + 256;
   const size_t need = state_end + (net ? 2 * vec + (size_t)N * sizeof(float) + 256 : 0);
   if (ps->scratch_bytes < need) {
     if (ps->scratch) (void)hipFree(ps->scratch);
@@ -414,7 +453,8 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   double* DX = xalt + 4 * N;
   double* DF = DX + (int64_t)m * N;
   double* part = DF + (int64_t)m * N;
-  AarState* S = (AarState*)(((uintptr_t)(part + (int64_t)nblk * ndmax) + 255) & ~(uintptr_t)255);
+  AarState* S =
+      (AarState*)(((uintptr_t)(part + (int64_t)std::max(nblk, nblk_norm) * ndmax) + 255) & ~(uintptr_t)255);
   double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
   double* dbuf = net ? r1buf + N : nullptr;
   float* ubuf = net ? (float*)(dbuf + N) : nullptr;
@@ -426,13 +466,17 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part);
   hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk_norm, S);
   const PoissonOp P = make_op(ps);
+  const int ntb = P.helical ? nblk : 0;  // theta blocks per launch
   AarState hs{};
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
+  // a Richardson iteration whose finalise has not been enqueued yet: it
+  // rides on the next theta launch, or runs alone before a poll
+  bool fin_pending = false;
+  auto fin_args = [&](int c) { return FinArgs{(const double*)part, nblk, c, max_iter, c & 1, tol, S}; };
   // iterations enqueued between polls of the device flag (the network's
   // kernels do not test it, so the learned path polls more often)
   const int chunk = net ? 6 : 16;
-  const dim3 grid = pgrid;
   while (count <= max_iter) {
     const int cend = std::min(max_iter, count + chunk - 1);
     for (; count <= cend; ++count) {
@@ -455,40 +499,49 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       A.xbuf = xb;
       A.tol = tol;
       A.omega = omega;
-      if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, A.xp, T, (const AarState*)S);
+      if (ntb > 0 || fin_pending)
+        hipLaunchKernelGGL(theta_kernel, dim3(ntb + (fin_pending ? 1 : 0)), dim3(PB), 0, st, P, A.xp, T, ntb,
+                           fin_args(count - 1));
+      fin_pending = false;
       if (!A.anderson) {
-        hipLaunchKernelGGL(aar_step_kernel<0>, grid, dim3(TX, TY), 0, st, P, A);
-        hipLaunchKernelGGL(aar_finalize_kernel<0>, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, 0, count,
-                           max_iter, xb, tol, S);
+        hipLaunchKernelGGL(aar_step_kernel<0>, dim3(nblk), dim3(PB), 0, st, P, A);
+        fin_pending = true;
       } else {
         switch (m) {
-          case 1: launch_anderson<1>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 2: launch_anderson<2>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 3: launch_anderson<3>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 4: launch_anderson<4>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 5: launch_anderson<5>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 6: launch_anderson<6>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          case 7: launch_anderson<7>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
-          default: launch_anderson<8>(grid, nblk, st, P, A, N, beta, X[xb ^ 1]); break;
+          case 1: launch_anderson<1>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 2: launch_anderson<2>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 3: launch_anderson<3>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 4: launch_anderson<4>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 5: launch_anderson<5>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 6: launch_anderson<6>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 7: launch_anderson<7>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          default: launch_anderson<8>(nblk, st, P, A, beta, X[xb ^ 1]); break;
         }
         if (net) {
           // learned correction of the Anderson iterate, with exact line search
           double* xn = X[xb ^ 1];
           const AarState* cS = S;
-          if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)xn, T, cS);
-          hipLaunchKernelGGL(resid_kernel, grid, dim3(TX, TY), 0, st, P, f, (const double*)xn, (const double*)T,
+          if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)xn, T, nblk,
+                                            FinArgs{});
+          hipLaunchKernelGGL(resid_kernel, dim3(nblk), dim3(PB), 0, st, P, f, (const double*)xn, (const double*)T,
                              r1buf, cS);
           int rc = unet_precondition(net, ps->coef, r1buf, ubuf);
           if (rc != CYX_OK) return rc;
-          hipLaunchKernelGGL(ls_dir_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)r1buf, (const float*)ubuf,
+          hipLaunchKernelGGL(ls_dir_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)r1buf, (const float*)ubuf,
                              dbuf, cS);
-          if (P.helical) hipLaunchKernelGGL(theta_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)dbuf, T, cS);
-          hipLaunchKernelGGL(ls_apply_kernel, grid, dim3(TX, TY), 0, st, P, (const double*)dbuf, (const double*)T,
+          if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)dbuf, T,
+                                            nblk, FinArgs{});
+          hipLaunchKernelGGL(ls_apply_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)dbuf, (const double*)T,
                              (const double*)r1buf, part, cS);
           hipLaunchKernelGGL(ls_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, S);
-          hipLaunchKernelGGL(ls_axpy_kernel, grid, dim3(TX, TY), 0, st, P, xn, (const double*)dbuf, cS);
+          hipLaunchKernelGGL(ls_axpy_kernel, dim3(nblk), dim3(PB), 0, st, P, xn, (const double*)dbuf, cS);
         }
       }
+    }
+    if (fin_pending) {
+      hipLaunchKernelGGL(theta_kernel, dim3(1), dim3(PB), 0, st, P, (const double*)X[(count - 1) & 1], T, 0,
+                         fin_args(count - 1));
+      fin_pending = false;
     }
     CYX_HIP(hipGetLastError());
     CYX_HIP(hipMemcpyAsync(&hs, S, sizeof(AarState), hipMemcpyDeviceToHost, st));
@@ -507,6 +560,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
     info->relres = hs.relres;
     info->converged = hs.done == 1;
     info->precond = net ? 1 : 0;
+    info->pinv_eig = hs.pinv_eig;
   }
   return hs.done == 1 ? CYX_OK : CYX_ERR_NOCONV;
 }

@@ -849,12 +849,18 @@ int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* 
   p->ctx = ctx;
   p->g = *grid;
   const std::vector<double> c = radial_coef(*grid, 1.0);
-  if (hipMalloc((void**)&p->coef, c.size() * sizeof(double)) != hipSuccess) {
+  std::vector<double> ct(c.size());
+  for (int i = 0; i < grid->nr; ++i)
+    for (int k = 0; k < RCOEF; ++k) ct[(size_t)k * grid->nr + i] = c[(size_t)i * RCOEF + k];
+  if (hipMalloc((void**)&p->coef, c.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&p->coefT, c.size() * sizeof(double)) != hipSuccess) {
+    (void)hipFree(p->coef);
     delete p;
     set_error("allocation failed");
     return CYX_ERR_NOMEM;
   }
   CYX_HIP(hipMemcpy(p->coef, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+  CYX_HIP(hipMemcpy(p->coefT, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice));
   *out = p;
   return CYX_OK;
 }
@@ -878,6 +884,7 @@ int cyx_poisson_destroy(cyx_poisson_t p) {
   if (!p) return CYX_OK;
   (void)hipDeviceSynchronize();
   (void)hipFree(p->coef);
+  (void)hipFree(p->coefT);
   (void)hipFree(p->scratch);
   (void)hipFree(p->tbuf.p);
   delete p;

@@ -249,7 +249,10 @@ typedef struct cyx_aar_info {
   int32_t precond;    /* 0 Jacobi; 1 learned (cyx_poisson_set_preconditioner);
                          2 learned, diverged, solved again with Jacobi (iterations
                          then count the Jacobi solve) */
-  int32_t reserved;
+  int32_t pinv_eig;   /* Anderson steps whose DF'DF was not certified full rank
+                         (lambda_min > 100 m trace eps) and took the
+                         eigen-decomposition pseudo-inverse; the others solve
+                         by Cholesky, which equals pinv there */
 } cyx_aar_info;
 
 /* Device operator for the Poisson problem: S.Lap_std at k = 0
