@@ -429,7 +429,8 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   rocblas_int* info = (rocblas_int*)(scal + ns);
   if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
   if ((rc = stencil_step(h, dtype, psi, nullptr, other, ns, N, 1.0, 0.0, 0.0)) != CYX_OK) return rc;
-  if ((rc = gram_w(h, dtype, psi, other, ns, N, Hs)) != CYX_OK) return rc;
+  // Hs in full: its two triangles are averaged below, as src/eigSolver.m:108 does
+  if ((rc = gram_rect(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, Hs, ns)) != CYX_OK) return rc;
   {

@@ -101,6 +101,12 @@ __device__ __forceinline__ cplx<R> cjfma(cplx<R> a, cplx<R> b, cplx<R> acc) {
   return cplx<R>{rfma(a.x, b.x, rfma(a.y, b.y, acc.x)), rfma(a.x, b.y, rfma(-a.y, b.x, acc.y))};
 }
 
+__device__ __forceinline__ double conjg(double a) { return a; }
+__device__ __forceinline__ float conjg(float a) { return a; }
+template <class R>
+__device__ __forceinline__ cplx<R> conjg(cplx<R> a) {
+  return cplx<R>{a.x, -a.y};
+}
 __device__ __forceinline__ double absq(double a) { return a * a; }
 __device__ __forceinline__ float absq(float a) { return a * a; }
 template <class R>

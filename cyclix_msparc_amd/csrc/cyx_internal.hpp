@@ -144,9 +144,10 @@ int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void
 
 // subspace kernels (FP64 accumulation)
 int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out /*dev*/);
-// M (na x nb, FP64 / complex FP64, column-major) = A^H diag(W) B over the N grid points
+// M (na x nb, FP64 / complex FP64, column-major) = A^H diag(W) B over the N grid points;
+// herm: the product is Hermitian (na == nb), only the upper tiles are computed and mirrored
 int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-              const double* W, void* M_out /*dev*/);
+              const double* W, void* M_out /*dev*/, bool herm = false);
 int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
 int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* scal /*dev ns*/);
 int hermitize(cyx_ham_s* h, int dtype, void* M, int ns);

@@ -1,145 +1,276 @@
 // Subspace kernels of the Rayleigh-Ritz step (src/eigSolver.m:99-124):
 // weighted Gram/projection M = A^H diag(W) B, rotation A*Q, column
 // W-normalisation.  All products accumulate in FP64 (FP32 inputs are
-// widened exactly, so the Gram of an FP32 block carries FP64 accuracy).
+// widened exactly, so the Gram of an FP32 block carries FP64 accuracy); the
+// Gram and the rotation run on the FP64 matrix cores.
 #include "cyx_internal.hpp"
 
 namespace cyx {
 
-constexpr int GT = 64;  // output tile
-constexpr int GK = 16;  // k chunk
+// Both products run on the FP64 matrix cores, v_mfma_f64_16x16x4_f64: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and receives D[4t + l/16][l%16] in
+// element t of its accumulator.  FP32 inputs are widened exactly on load.
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4d mfma64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
-// Partial[kb][a][b] = sum_{n in chunk kb} conj(A[n,a]) W[n] B[n,b]  (column-major na x nb per partial)
+// ---- Gram M = A^H diag(W) B.  A wave owns a 32 x 32 output tile (2 x 2
+// MFMA tiles) over a k sub-range; the 4 waves of a block take consecutive
+// sub-ranges of one tile and sum them in LDS, one partial per block.
+// The k slots of an MFMA may hold any points as long as A and B agree, so
+// lane (g, c) takes points n0 + SP g + s (s < SP) of column c: its loads run
+// along a column, the 4 lane groups of a column reading one 128-byte line per
+// step.  The k loop is software-pipelined: the loads of step k+1 are issued
+// before the MFMAs of step k (scheduling barriers keep that order), which
+// the compiler's own schedule does not do.  Hermitian products compute the
+// tiles with ta <= tb only.
+constexpr int GS = 32;  // output tile of a wave
 template <class T>
-__global__ void __launch_bounds__(256) gram_partial_kernel(const T* __restrict__ A, int64_t lda, int na,
-                                                           const T* __restrict__ B, int64_t ldb, int nb,
-                                                           const double* __restrict__ W, int64_t N, int64_t krange,
-                                                           typename acc_of<T>::type* __restrict__ part) {
+constexpr int gram_sp() {  // points per lane per k step
+  return sizeof(T) >= 16 ? 2 : (sizeof(T) >= 8 ? 4 : 8);
+}
+
+__device__ inline void gram_tile_of(int tp, bool herm, int nta, int& ta, int& tb) {
+  if (herm) {
+    tb = 0;
+    while ((tb + 1) * (tb + 2) / 2 <= tp) ++tb;
+    ta = tp - tb * (tb + 1) / 2;
+  } else {
+    ta = tp % nta;
+    tb = tp / nta;
+  }
+}
+
+template <class T, int SP>
+struct GramStep {
+  T a[2][SP], b[2][SP];
+  double w[SP];
+};
+
+template <class T, int SP>
+__device__ __forceinline__ void gram_load(GramStep<T, SP>& S, const T* __restrict__ A, int64_t lda, int na,
+                                          const T* __restrict__ B, int64_t ldb, int nb,
+                                          const double* __restrict__ W, int64_t n0, int64_t k1, const int (&ca)[2],
+                                          const int (&cb)[2], int g) {
+#pragma unroll
+  for (int s = 0; s < SP; ++s) {
+    const int64_t n = n0 + SP * g + s;
+    const bool ok = n < k1;
+    S.w[s] = ok ? W[n] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      S.a[i][s] = (ok && ca[i] < na) ? A[(int64_t)ca[i] * lda + n] : zero<T>();
+      S.b[i][s] = (ok && cb[i] < nb) ? B[(int64_t)cb[i] * ldb + n] : zero<T>();
+    }
+  }
+}
+
+template <class T, int SP, int NC>
+__device__ __forceinline__ void gram_mfma(const GramStep<T, SP>& S, v4d (&acc)[NC][2][2]) {
+#pragma unroll
+  for (int s = 0; s < SP; ++s) {
+    if constexpr (traits<T>::is_complex) {
+      // conj(a) w b: re += ar br + ai bi, im += ar bi - ai br
+      cplx<double> av[2];
+      double br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) av[i] = widen(S.a[i][s]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const cplx<double> bv = widen(S.b[j][s]);
+        br[j] = S.w[s] * bv.x;
+        bi[j] = S.w[s] * bv.y;
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[0][i][j] = mfma64(av[i].x, br[j], acc[0][i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[1][i][j] = mfma64(av[i].x, bi[j], acc[1][i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[0][i][j] = mfma64(av[i].y, bi[j], acc[0][i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[1][i][j] = mfma64(-av[i].y, br[j], acc[1][i][j]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[0][i][j] = mfma64(widen(S.a[i][s]), S.w[s] * widen(S.b[j][s]), acc[0][i][j]);
+    }
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256, 2) gram_mfma_kernel(const T* __restrict__ A, int64_t lda, int na,
+                                                        const T* __restrict__ B, int64_t ldb, int nb,
+                                                        const double* __restrict__ W, int64_t N, int64_t chunk,
+                                                        int herm, int nta, typename acc_of<T>::type* __restrict__ part) {
   using D = typename acc_of<T>::type;
-  __shared__ D sa[GK][GT + 1];
-  __shared__ D sb[GK][GT + 1];
-  const int ta = blockIdx.x, tb = blockIdx.y, kb = blockIdx.z;
-  const int a0 = ta * GT, b0 = tb * GT;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  D acc[4][4];
+  constexpr bool CX = traits<T>::is_complex;
+  constexpr int NC = CX ? 2 : 1;  // accumulator sets: (re, im) or one
+  constexpr int SP = gram_sp<T>();
+  __shared__ double red[4][NC][2][2][4][64];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  int ta, tb;
+  gram_tile_of(blockIdx.x, herm != 0, nta, ta, tb);
+  const int a0 = ta * GS, b0 = tb * GS;
+  const int64_t k0 = ((int64_t)blockIdx.y * 4 + wv) * chunk, k1 = min(N, k0 + chunk);
+  v4d acc[NC][2][2];
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+  for (int q = 0; q < NC; ++q)
 #pragma unroll
-    for (int y = 0; y < 4; ++y) acc[x][y] = D{};
-  const int64_t k0 = (int64_t)kb * krange;
-  const int64_t k1 = min(N, k0 + krange);
-  // loader mapping: 256 threads load GK x GT = 1024 elements of each operand, 4 each
-  for (int64_t kc = k0; kc < k1; kc += GK) {
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int idx = threadIdx.x + 256 * l;
-      const int kk = idx % GK;
-      const int col = idx / GK;
-      const int64_t n = kc + kk;
-      D va{}, vb{};
-      if (n < k1) {
-        const double w = W[n];
-        if (a0 + col < na) va = widen(A[(int64_t)(a0 + col) * lda + n]);
-        if (b0 + col < nb) {
-          const D bb = widen(B[(int64_t)(b0 + col) * ldb + n]);
-          vb = rmul(w, bb);
+      for (int j = 0; j < 2; ++j) acc[q][i][j] = v4d{0, 0, 0, 0};
+  const int ca[2] = {a0 + c, a0 + 16 + c}, cb[2] = {b0 + c, b0 + 16 + c};
+  if (k0 < k1) {
+    GramStep<T, SP> S0, S1;
+    gram_load(S0, A, lda, na, B, ldb, nb, W, k0, k1, ca, cb, g);
+    for (int64_t n0 = k0;;) {
+      const bool more = n0 + 4 * SP < k1;
+      if (more) gram_load(S1, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g);
+      __builtin_amdgcn_sched_barrier(0);
+      gram_mfma(S0, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      if (!more) break;
+      n0 += 4 * SP;
+      const bool more2 = n0 + 4 * SP < k1;
+      if (more2) gram_load(S0, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g);
+      __builtin_amdgcn_sched_barrier(0);
+      gram_mfma(S1, acc);
+      __builtin_amdgcn_sched_barrier(0);
+      if (!more2) break;
+      n0 += 4 * SP;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) red[wv][q][i][j][t][l] = acc[q][i][j][t];
+  __syncthreads();
+  // the 32 x 32 tile: element e -> (i, j, t, lane) of the MFMA layout, the 4 waves summed in order
+  D* out = part + ((int64_t)blockIdx.x * gridDim.y + blockIdx.y) * (GS * GS);
+  for (int e = threadIdx.x; e < GS * GS; e += 256) {
+    const int ll = e & 63, t = (e >> 6) & 3, j = (e >> 8) & 1, i = (e >> 9) & 1;
+    const int al = 16 * i + 4 * t + (ll >> 4), bl = 16 * j + (ll & 15);
+    D v{};
+    if constexpr (CX) {
+      double sr = 0, si = 0;
+      for (int w2 = 0; w2 < 4; ++w2) {
+        sr += red[w2][0][i][j][t][ll];
+        si += red[w2][1][i][j][t][ll];
+      }
+      v = D{sr, si};
+    } else {
+      double sr = 0;
+      for (int w2 = 0; w2 < 4; ++w2) sr += red[w2][0][i][j][t][ll];
+      v = sr;
+    }
+    out[bl * GS + al] = v;
+  }
+}
+
+// M (na x nb column-major) = fixed-order sum of the block partials; Hermitian
+// products keep the upper triangle and mirror it (conjugated) into the lower
+template <class D>
+__global__ void gram_reduce_kernel(const D* __restrict__ part, int nchb, int herm, int nta, int na, int nb,
+                                   D* __restrict__ M) {
+  const int tp = blockIdx.x, e = threadIdx.x + blockDim.x * blockIdx.y;
+  if (e >= GS * GS) return;
+  int ta, tb;
+  gram_tile_of(tp, herm != 0, nta, ta, tb);
+  const int a = ta * GS + (e % GS), b = tb * GS + (e / GS);
+  D s{};
+  for (int k = 0; k < nchb; ++k) s = add(s, part[((int64_t)tp * nchb + k) * (GS * GS) + e]);
+  if (a < na && b < nb) {
+    if (!herm) {
+      M[(int64_t)b * na + a] = s;
+    } else if (a <= b) {  // diagonal tiles hold both triangles: keep the upper one
+      M[(int64_t)b * na + a] = s;
+      if (a != b) M[(int64_t)a * na + b] = conjg(s);
+    }
+  }
+}
+
+// ---- rotation out = A Q, computed as out^T = Q^T A^T so that lane (g, c)
+// holds out[n0 + c][b0 + 4t + g]: 16 consecutive points of one column per
+// store.  A block of RW waves covers 16 RW points and all ns columns in
+// passes of 64 (4 MFMA tiles); Q[a chunk][64 columns] is staged in LDS.
+constexpr int RW = 8;    // waves per rotation block
+constexpr int RAC = 16;  // a per LDS chunk
+
+template <class T>
+__global__ void __launch_bounds__(RW * 64) rotate_mfma_kernel(const T* __restrict__ A,
+                                                              const typename acc_of<T>::type* __restrict__ Q,
+                                                              T* __restrict__ out, int64_t N, int64_t ld, int ns) {
+  using D = typename acc_of<T>::type;
+  constexpr bool CX = traits<T>::is_complex;
+  constexpr int NC = CX ? 2 : 1;
+  __shared__ D sq[RAC][64 + 1];  // [a][b]
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+  const int64_t n = ((int64_t)blockIdx.x * RW + wv) * 16 + c;
+  const bool nok = n < N;
+  for (int bg = 0; bg < ns; bg += 64) {
+    v4d acc[NC][4];
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+      for (int bt = 0; bt < 4; ++bt) acc[q][bt] = v4d{0, 0, 0, 0};
+    for (int ac = 0; ac < ns; ac += RAC) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < RAC * 64; e += RW * 64) {
+        const int aa = e % RAC, bb = e / RAC;
+        sq[aa][bb] = (ac + aa < ns && bg + bb < ns) ? Q[(int64_t)(bg + bb) * ns + ac + aa] : D{};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k4 = 0; k4 < RAC; k4 += 4) {
+        const int a = ac + k4 + g;
+        const D x = (nok && a < ns) ? widen(A[(int64_t)a * ld + n]) : D{};
+#pragma unroll
+        for (int bt = 0; bt < 4; ++bt) {
+          const D qv = sq[k4 + g][bt * 16 + c];
+          if constexpr (CX) {
+            // x q: re += xr qr - xi qi, im += xi qr + xr qi
+            acc[0][bt] = mfma64(qv.x, x.x, acc[0][bt]);
+            acc[1][bt] = mfma64(qv.x, x.y, acc[1][bt]);
+            acc[0][bt] = mfma64(-qv.y, x.y, acc[0][bt]);
+            acc[1][bt] = mfma64(qv.y, x.x, acc[1][bt]);
+          } else {
+            acc[0][bt] = mfma64(qv, x, acc[0][bt]);
+          }
         }
       }
-      sa[kk][col] = va;
-      sb[kk][col] = vb;
     }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < GK; ++kk) {
-      D av[4], bv[4];
+    if (nok) {
 #pragma unroll
-      for (int x = 0; x < 4; ++x) av[x] = sa[kk][tx + 16 * x];
+      for (int bt = 0; bt < 4; ++bt)
 #pragma unroll
-      for (int y = 0; y < 4; ++y) bv[y] = sb[kk][ty + 16 * y];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = cjfma(av[x], bv[y], acc[x][y]);
+        for (int t = 0; t < 4; ++t) {
+          const int b = bg + 16 * bt + 4 * t + g;
+          if (b < ns) {
+            D v;
+            if constexpr (CX)
+              v = D{acc[0][bt][t], acc[1][bt][t]};
+            else
+              v = acc[0][bt][t];
+            out[(int64_t)b * ld + n] = narrow<T>(v);
+          }
+        }
     }
-    __syncthreads();
   }
-  D* out = part + (int64_t)kb * na * nb;
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int a = a0 + tx + 16 * x, b = b0 + ty + 16 * y;
-      if (a < na && b < nb) out[(int64_t)b * na + a] = acc[x][y];
-    }
-}
-
-template <class D>
-__global__ void reduce_partials_kernel(const D* __restrict__ part, int nparts, int64_t len, D* __restrict__ out) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= len) return;
-  D s{};
-  for (int k = 0; k < nparts; ++k) s = add(s, part[(int64_t)k * len + e]);
-  out[e] = s;
-}
-
-// out[n,b] = sum_a A[n,a] Q[a,b]   (Q: ns x ns column-major, FP64 / complex FP64)
-template <class T>
-__global__ void __launch_bounds__(256) rotate_kernel(const T* __restrict__ A, const typename acc_of<T>::type* __restrict__ Q,
-                                                     T* __restrict__ out, int64_t N, int64_t ld, int ns) {
-  using D = typename acc_of<T>::type;
-  __shared__ D sa[GK][GT + 1];  // [a][n]
-  __shared__ D sq[GK][GT + 1];  // [a][b]
-  const int64_t n0 = (int64_t)blockIdx.x * GT;
-  const int b0 = blockIdx.y * GT;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  D acc[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) acc[x][y] = D{};
-  for (int ac = 0; ac < ns; ac += GK) {
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int idx = threadIdx.x + 256 * l;
-      // A tile: n fastest
-      {
-        const int nn = idx % GT, aa = idx / GT;
-        const int64_t n = n0 + nn;
-        D v{};
-        if (n < N && ac + aa < ns) v = widen(A[(int64_t)(ac + aa) * ld + n]);
-        sa[aa][nn] = v;
-      }
-      // Q tile: a fastest
-      {
-        const int aa = idx % GK, bb = idx / GK;
-        D v{};
-        if (ac + aa < ns && b0 + bb < ns) v = Q[(int64_t)(b0 + bb) * ns + ac + aa];
-        sq[aa][bb] = v;
-      }
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < GK; ++kk) {
-      D av[4], qv[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x) av[x] = sa[kk][tx + 16 * x];
-#pragma unroll
-      for (int y = 0; y < 4; ++y) qv[y] = sq[kk][ty + 16 * y];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = cfma(av[x], qv[y], acc[x][y]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int64_t n = n0 + tx + 16 * x;
-      const int b = b0 + ty + 16 * y;
-      if (n < N && b < ns) out[(int64_t)b * ld + n] = narrow<T>(acc[x][y]);
-    }
 }
 
 // per-column 1/sqrt(sum W |a|^2)   (src/eigSolver.m:122)
@@ -191,45 +322,49 @@ __global__ void hermitize_kernel(D* M, int ns) {
 // ---------------------------------------------------------------- host side
 template <class T>
 static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-                     const double* W, void* M_out) {
+                     const double* W, void* M_out, bool herm) {
   using D = typename acc_of<T>::type;
   const int64_t N = h->N();
-  const int ta = ceil_div(na, GT), tb = ceil_div(nb, GT);
-  int kparts = ceil_div(2048, ta * tb);
-  int64_t krange = (N + kparts - 1) / kparts;
-  krange = ((krange + GK - 1) / GK) * GK;
-  kparts = (int)((N + krange - 1) / krange);
-  const size_t pbytes = (size_t)kparts * na * nb * sizeof(D);
+  herm = herm && na == nb;
+  const int nta = ceil_div(na, GS), ntb = ceil_div(nb, GS);
+  const int ntp = herm ? nta * (nta + 1) / 2 : nta * ntb;
+  // about 8192 waves in all (4 per block), k sub-ranges in whole k steps
+  int nchb = std::max(1, ceil_div(8192, 4 * ntp));
+  const int kstep = 4 * gram_sp<T>();
+  int64_t chunk = ceil_div(N, 4 * (int64_t)nchb);
+  chunk = ceil_div(chunk, kstep) * kstep;
+  nchb = (int)ceil_div(N, 4 * chunk);
+  const size_t pbytes = (size_t)ntp * nchb * GS * GS * sizeof(D);
   CYX_CHECK(ensure(h->small, pbytes) == CYX_OK, "partials allocation failed");
   hipStream_t st = h->ctx->stream;
   PhaseTimer tm(h->ctx, PH_GRAM);
-  hipLaunchKernelGGL(gram_partial_kernel<T>, dim3(ta, tb, kparts), dim3(256), 0, st, (const T*)A, lda, na,
-                     (const T*)B, ldb, nb, W, N, krange, (D*)h->small.p);
+  hipLaunchKernelGGL(gram_mfma_kernel<T>, dim3(ntp, nchb), dim3(256), 0, st, (const T*)A, lda, na, (const T*)B, ldb,
+                     nb, W, N, chunk, herm ? 1 : 0, nta, (D*)h->small.p);
   CYX_HIP(hipGetLastError());
-  const int64_t len = (int64_t)na * nb;
-  hipLaunchKernelGGL(reduce_partials_kernel<D>, dim3(ceil_div(len, 256)), dim3(256), 0, st, (const D*)h->small.p,
-                     kparts, len, (D*)M_out);
+  hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / 256), dim3(256), 0, st, (const D*)h->small.p, nchb,
+                     herm ? 1 : 0, nta, na, nb, (D*)M_out);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
 
 int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-              const double* W, void* M_out) {
+              const double* W, void* M_out, bool herm) {
   switch (dtype) {
     case CYX_F64:
-      return gram_impl<double>(h, A, lda, na, B, ldb, nb, W, M_out);
+      return gram_impl<double>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
     case CYX_F32:
-      return gram_impl<float>(h, A, lda, na, B, ldb, nb, W, M_out);
+      return gram_impl<float>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
     case CYX_C128:
-      return gram_impl<cplx<double>>(h, A, lda, na, B, ldb, nb, W, M_out);
+      return gram_impl<cplx<double>>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
     case CYX_C64:
-      return gram_impl<cplx<float>>(h, A, lda, na, B, ldb, nb, W, M_out);
+      return gram_impl<cplx<float>>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
   }
   return CYX_ERR_ARG;
 }
 
 int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out) {
-  return gram_rect(h, dtype, A, ld, ns, B, ld, ns, h->Wm, M_out);
+  // A^H W A is Hermitian by construction; a product of two blocks is not
+  return gram_rect(h, dtype, A, ld, ns, B, ld, ns, h->Wm, M_out, A == B);
 }
 
 template <class T>
@@ -237,7 +372,7 @@ static int rotate_impl(cyx_ham_s* h, const void* A, const void* Q, void* out, in
   using D = typename acc_of<T>::type;
   const int64_t N = h->N();
   PhaseTimer tm(h->ctx, PH_ROTATE);
-  hipLaunchKernelGGL(rotate_kernel<T>, dim3(ceil_div(N, GT), ceil_div(ns, GT)), dim3(256), 0, h->ctx->stream,
+  hipLaunchKernelGGL(rotate_mfma_kernel<T>, dim3(ceil_div(N, (int64_t)RW * 16)), dim3(RW * 64), 0, h->ctx->stream,
                      (const T*)A, (const D*)Q, (T*)out, N, ld, ns);
   CYX_HIP(hipGetLastError());
   return CYX_OK;

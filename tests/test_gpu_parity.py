@@ -133,6 +133,36 @@ def test_eigsolver_pass_parity(device, name):
     assert np.allclose(ev, ev_ref, rtol=1e-9, atol=1e-9)
 
 
+@pytest.mark.parametrize("name,dt", [("cyclic_gamma", "f64"), ("cyclic_gamma", "f32"),
+                                     ("helical_bloch", "c128"), ("helical_bloch", "c64")])
+def test_rayleigh_ritz_many_states(device, name, dt):
+    """Rayleigh-Ritz on 70 states: the Gram products cover several 64 x 64
+    output regions (Hermitian overlap: upper regions mirrored; H projection:
+    all regions) with a partial last region, the rotation several column
+    passes.  Eigenvalues and the spanned subspace follow the oracle's metric
+    Rayleigh-Ritz (src/eigSolver.m:99-124) on the same (quantised) block."""
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    N, ns = sysm.grid.N, 70
+    X = rand_block(N, ns, dt, 11)
+    Xq = X.astype(NP[dt]).astype(X.dtype)
+    ev_ref, psi_ref = O.rayleigh_ritz_w(oham, Xq)
+    psi_d = to_dev(Xq, dt, device)
+    ev = ham.rayleigh_ritz(psi_d)
+    psi = to_host(psi_d)
+    tol = 1e-9 if dt in ("f64", "c128") else 2e-5
+    err = np.max(np.abs(ev - ev_ref)) / np.max(np.abs(ev_ref))
+    print(f"rayleigh-ritz {name} {dt} ns={ns}: eigenvalue rel err {err:.3e}", flush=True)
+    assert err < tol
+    Wm = O.metric_weights(oham.g)
+    V = np.random.default_rng(5).standard_normal((N, 4))
+    assert rel(_subspace_projector(psi, Wm, V), _subspace_projector(psi_ref, Wm, V)) < (1e-8 if tol < 1e-6 else 1e-3)
+    nrm = np.sum(sysm.W[:, None] * np.abs(psi) ** 2, axis=0)
+    assert np.allclose(nrm, 1.0, atol=1e-12 if tol < 1e-6 else 1e-5)
+    ham.close()
+
+
 @pytest.mark.parametrize("kpt,twist", [((0, 0, 0), 0.0), ((0, 1.0, 0.6), 0.05)])
 def test_chefsi_converges_to_dense_eigenvalues(device, kpt, twist):
     """Repeated device passes reach the lowest eigenvalues of the dense H."""
