@@ -11,7 +11,10 @@
 //        fixed-order sum of its partials, relres and the convergence flag;
 //     2. step kernel: res1 = f - Lap x, res = res1 ./ diag, history columns,
 //        x_new = x + omega res (written speculatively into the other x
-//        buffer), per-block partials of ||res1||^2.
+//        buffer), per-block partials of ||res1||^2.  A step block takes 8
+//        z planes of one 64-point plane segment and stages x (with its z and
+//        r halo), T and the radial coefficients in LDS, so only the theta taps
+//        are read through the vector cache.
 //   Anderson step (four launches): theta (+ finalise of the previous
 //   iteration), step kernel with the DF'DF and DF'res partials, a one-block
 //   finalize that also computes gamma = pinv(DF'DF) DF'res on one wavefront,
@@ -38,7 +41,8 @@ struct AarState {
 struct PoissonOp {
   int nr, nt, nz, N;
   int helical;
-  const double* coefT;  // [RCOEF][nr]: c+[6], c-[6], c_theta, diag (Lap_std, no -1/2), radius fastest
+  const double* coefT;  // [RCOEF][nr]: c+[6], c-[6], c_theta, diag (Lap_std, no -1/2), radius fastest;
+                        // radial taps outside [0, nr) are zero
   double w2[FDN + 1];   // w2_p (theta second derivative, times c_theta per radius)
   double wz2[FDN + 1];  // w2_q / dz^2
   double wt1[FDN + 1];  // -2 tau w1_p / dt
@@ -230,34 +234,97 @@ struct StepArgs {
   double tol, omega;
 };
 
-// MH: compile-time history size of the Anderson dot products (0: Richardson
-// step, which writes x_new; MH > 0: Anderson step, which writes the partials
-// of DF'DF and DF'res instead)
+// Step kernel.  MH: compile-time history size of the Anderson dot products
+// (0: Richardson step, which writes x_new; MH > 0: Anderson step, which
+// writes the partials of DF'DF and DF'res instead).
+// Brick form: block b covers plane segment seg (64
+// consecutive points of a plane in the r-fastest order) of planes
+// 8 zc .. 8 zc + 7, wave w one plane.  LDS holds x at planes k - FDN ..
+// k + FDN of the block's planes over the segment and an r halo of FDN on
+// each side, T at the same planes over the segment, and the 14 coefficients
+// of the segment's radii.  Halo positions outside the plane are clamped into
+// it: they only meet radial taps whose coefficient is zero.
+constexpr int BZ = PB / 64;           // planes per step block
+constexpr int BSEG = 64;              // points per plane segment
+constexpr int BZH = BZ + 2 * FDN;     // staged planes
+constexpr int BXW = BSEG + 2 * FDN;   // staged x row with r halo
+
+struct BrickLds {
+  double x[BZH][BXW];
+  double t[BZH][BSEG];
+  double c[RCOEF][BSEG];
+};
+
 template <int MH>
-__global__ void __launch_bounds__(PB) aar_step_kernel(PoissonOp P, StepArgs A) {
+__global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArgs A) {
   constexpr int ND = MH * (MH + 1) / 2 + MH;
   __shared__ double red[(PB / 64) * (1 + ND)];
+  __shared__ BrickLds L;
   const int done = A.S->done;  // stores are skipped after convergence; the loads need not wait for it
   const int64_t N = P.N;
+  const int nrt = P.nr * P.nt;
+  const int nseg = (nrt + BSEG - 1) / BSEG;
+  const int seg = blockIdx.x % nseg, k0 = (blockIdx.x / nseg) * BZ;
+  const int p0 = seg * BSEG;
+  for (int e = threadIdx.x; e < BZH * BXW; e += PB) {
+    const int pl = e / BXW, off = e % BXW;
+    const int k = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
+    const int pos = min(max(p0 - FDN + off, 0), nrt - 1);
+    L.x[pl][off] = A.xp[k * nrt + pos];
+  }
+  if (P.helical) {
+    for (int e = threadIdx.x; e < BZH * BSEG; e += PB) {
+      const int pl = e / BSEG, off = e % BSEG;
+      const int k = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
+      const int pos = min(p0 + off, nrt - 1);
+      L.t[pl][off] = A.T[k * nrt + pos];
+    }
+  }
+  for (int e = threadIdx.x; e < RCOEF * BSEG; e += PB) {
+    const int cc = e / BSEG, off = e % BSEG;
+    const int pos = min(p0 + off, nrt - 1);
+    L.c[cc][off] = P.coefT[cc * P.nr + pos % P.nr];
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = k0 + w, pos = p0 + lane;
+  const bool ok = k < P.nz && pos < nrt;
+  const int j = pos / P.nr, i = pos - j * P.nr;
+  const int n = k * nrt + pos;
+  __syncthreads();
   double v[1 + ND];
 #pragma unroll
   for (int d = 0; d < 1 + ND; ++d) v[d] = 0;
-  const Pt q = point_of(P);
   double x = 0, r = 0, xold = 0, rold = 0;
-  if (q.ok) {
-    const int n = q.n;
-    x = A.xp[n];
-    const double r1 = A.f[n] - lap_at(P, A.xp, A.T, q);
+  if (ok) {
+    const int xc = FDN + lane, zc = FDN + w;
+    x = L.x[zc][xc];
+    double acc = L.c[2 * FDN + 1][lane] * x;
+#pragma unroll
+    for (int p = 1; p <= FDN; ++p) {
+      acc = __builtin_fma(L.c[p - 1][lane], L.x[zc][xc + p], acc);
+      acc = __builtin_fma(L.c[FDN + p - 1][lane], L.x[zc][xc - p], acc);
+    }
+    const int rowk = k * nrt + i;
+    double st = 0.0;
+#pragma unroll
+    for (int p = 1; p <= FDN; ++p)
+      st = __builtin_fma(P.w2[p], A.xp[rowk + wrapi(j + p, P.nt) * P.nr] + A.xp[rowk + wrapi(j - p, P.nt) * P.nr], st);
+    acc = __builtin_fma(L.c[2 * FDN][lane], st, acc);
+#pragma unroll
+    for (int q = 1; q <= FDN; ++q) {
+      acc = __builtin_fma(P.wz2[q], L.x[zc + q][xc] + L.x[zc - q][xc], acc);
+      if (P.helical) acc = __builtin_fma(P.wz1[q], L.t[zc + q][lane] - L.t[zc - q][lane], acc);
+    }
+    const double r1 = A.f[n] - acc;
     v[0] = r1 * r1;
-    r = r1 / diag_at(P, q.i);
+    r = r1 / L.c[2 * FDN + 1][lane];
     if (A.hcol >= 0) {
       xold = A.xo[n];
       rold = A.fold[n];
     }
   }
   if (done) return;
-  if (q.ok) {
-    const int n = q.n;
+  if (ok) {
     A.res[n] = r;
     if (A.hcol >= 0) {
       A.DX[(int64_t)A.hcol * N + n] = x - xold;
@@ -361,11 +428,16 @@ int poisson_rhs(cyx_poisson_s* ps, const double* rho, const double* b, double* f
   return CYX_OK;
 }
 
+// step blocks (brick form) and point blocks (one point per thread)
+static int step_blocks(const PoissonOp& P) {
+  return ceil_div(P.nr * P.nt, BSEG) * ceil_div(P.nz, BZ);
+}
+
 template <int MH>
-static void launch_anderson(int nblk, hipStream_t st, const PoissonOp& P, const StepArgs& A, double beta,
+static void launch_anderson(int nblk, int nstep, hipStream_t st, const PoissonOp& P, const StepArgs& A, double beta,
                             double* xnew) {
-  hipLaunchKernelGGL(aar_step_kernel<MH>, dim3(nblk), dim3(PB), 0, st, P, A);
-  hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nblk, A.count,
+  hipLaunchKernelGGL(aar_step_brick_kernel<MH>, dim3(nstep), dim3(PB), 0, st, P, A);
+  hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nstep, A.count,
                      A.max_iter, A.xbuf, A.tol, A.S);
   hipLaunchKernelGGL(anderson_update_kernel<MH>, dim3(nblk), dim3(PB), 0, st, P, A.xp, (const double*)A.res, xnew,
                      (const double*)A.DX, (const double*)A.DF, beta, (const AarState*)A.S);
@@ -431,11 +503,13 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   hipStream_t st = ps->ctx->stream;
   const int nblk = (int)ceil_div(N, PB);
   const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
+  const int nstep = step_blocks(make_op(ps));
   const int ndmax = 1 + m * (m + 1) / 2 + m;
   // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
   const size_t vec = (size_t)N * sizeof(double);
   const size_t state_end = vec * (4 + 2 * (size_t)m) +
-                           (size_t)std::max(nblk, nblk_norm) * ndmax * sizeof(double) + sizeof(AarState) + 256;
+                           (size_t)std::max({nblk, nblk_norm, nstep}) * ndmax * sizeof(double) + sizeof(AarState) +
+                           256;
   const size_t need = state_end + (net ? 2 * vec + (size_t)N * sizeof(float) + 256 : 0);
   if (ps->scratch_bytes < need) {
     if (ps->scratch) (void)hipFree(ps->scratch);
@@ -454,7 +528,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   double* DF = DX + (int64_t)m * N;
   double* part = DF + (int64_t)m * N;
   AarState* S =
-      (AarState*)(((uintptr_t)(part + (int64_t)std::max(nblk, nblk_norm) * ndmax) + 255) & ~(uintptr_t)255);
+      (AarState*)(((uintptr_t)(part + (int64_t)std::max({nblk, nblk_norm, nstep}) * ndmax) + 255) & ~(uintptr_t)255);
   double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
   double* dbuf = net ? r1buf + N : nullptr;
   float* ubuf = net ? (float*)(dbuf + N) : nullptr;
@@ -473,7 +547,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   // a Richardson iteration whose finalise has not been enqueued yet: it
   // rides on the next theta launch, or runs alone before a poll
   bool fin_pending = false;
-  auto fin_args = [&](int c) { return FinArgs{(const double*)part, nblk, c, max_iter, c & 1, tol, S}; };
+  auto fin_args = [&](int c) { return FinArgs{(const double*)part, nstep, c, max_iter, c & 1, tol, S}; };
   // iterations enqueued between polls of the device flag (the network's
   // kernels do not test it, so the learned path polls more often)
   const int chunk = net ? 6 : 16;
@@ -504,18 +578,18 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
                            fin_args(count - 1));
       fin_pending = false;
       if (!A.anderson) {
-        hipLaunchKernelGGL(aar_step_kernel<0>, dim3(nblk), dim3(PB), 0, st, P, A);
+        hipLaunchKernelGGL(aar_step_brick_kernel<0>, dim3(nstep), dim3(PB), 0, st, P, A);
         fin_pending = true;
       } else {
         switch (m) {
-          case 1: launch_anderson<1>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 2: launch_anderson<2>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 3: launch_anderson<3>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 4: launch_anderson<4>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 5: launch_anderson<5>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 6: launch_anderson<6>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          case 7: launch_anderson<7>(nblk, st, P, A, beta, X[xb ^ 1]); break;
-          default: launch_anderson<8>(nblk, st, P, A, beta, X[xb ^ 1]); break;
+          case 1: launch_anderson<1>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 2: launch_anderson<2>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 3: launch_anderson<3>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 4: launch_anderson<4>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 5: launch_anderson<5>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 6: launch_anderson<6>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 7: launch_anderson<7>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          default: launch_anderson<8>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
         }
         if (net) {
           // learned correction of the Anderson iterate, with exact line search

@@ -851,8 +851,16 @@ int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* 
   p->g = *grid;
   const std::vector<double> c = radial_coef(*grid, 1.0);
   std::vector<double> ct(c.size());
+  // radius-fastest copy for the AAR kernels, with the radial taps that fall
+  // outside [0, nr) set to zero (the stencil is truncated there), so the
+  // kernels apply every tap without a branch
   for (int i = 0; i < grid->nr; ++i)
-    for (int k = 0; k < RCOEF; ++k) ct[(size_t)k * grid->nr + i] = c[(size_t)i * RCOEF + k];
+    for (int k = 0; k < RCOEF; ++k) {
+      double v = c[(size_t)i * RCOEF + k];
+      if (k < FDN && i + k + 1 >= grid->nr) v = 0.0;               // c+[p], p = k + 1
+      if (k >= FDN && k < 2 * FDN && i - (k - FDN + 1) < 0) v = 0.0;  // c-[p], p = k - FDN + 1
+      ct[(size_t)k * grid->nr + i] = v;
+    }
   if (hipMalloc((void**)&p->coef, c.size() * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&p->coefT, c.size() * sizeof(double)) != hipSuccess) {
     (void)hipFree(p->coef);

@@ -125,7 +125,7 @@ struct cyx_poisson_s {
   cyx_unet_s* net = nullptr;  // learned preconditioner (null: Jacobi only)
   cyx_grid_desc g{};
   double* coef = nullptr;   // [nr][RCOEF] Lap_std coefficients (no -1/2)
-  double* coefT = nullptr;  // [RCOEF][nr] the same coefficients, radius fastest (coalesced loads)
+  double* coefT = nullptr;  // [RCOEF][nr] the same, radius fastest, taps outside [0, nr) zeroed
   double* scratch = nullptr;
   size_t scratch_bytes = 0;
   cyx::DeviceBuf tbuf;      // theta-derivative buffer of cyx_poisson_apply
