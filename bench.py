@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per recurrence launch of the fused stencil at this workload, from
 # the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
 # calibration) recorded in profiles/scf_step_cnt10_helical.md
-MEASURED_TRAFFIC = {("cnt10_helical", 16): 4.2047e9, ("cnt10_helical", 8): 2.1830e9}
+MEASURED_TRAFFIC = {("cnt10_helical", 16): 4.2697e+09, ("cnt10_helical", 8): 2.1971e+09}
 METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM peak"
 
 
