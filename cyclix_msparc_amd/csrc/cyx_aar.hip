@@ -84,11 +84,10 @@ struct FinArgs {
 
 __device__ void finalize_richardson(const FinArgs& F) {
   if (F.S->done) return;
-  __shared__ double tot[1];
-  column_sums(F.part, F.nblk, 1, tot);
-  __syncthreads();
+  __shared__ double red[PB / 64];
+  const double tot = column_sums_all<1, PB>(F.part, F.nblk, red);
   if (threadIdx.x == 0) {
-    const double relres = sqrt(tot[0]) * F.S->nb_inv;
+    const double relres = sqrt(tot) * F.S->nb_inv;
     F.S->relres = relres;
     F.S->iters = F.count;
     F.S->final_buf = F.xbuf;
@@ -360,9 +359,13 @@ __global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, in
   if (S->done) return;
   constexpr int ND = MH * (MH + 1) / 2 + MH;
   __shared__ double tot[ND + 1];
+  __shared__ double red[(AB / 64) * (ND + 1)];
   __shared__ PinvShared<MH> W;
   __shared__ int go;
-  column_sums(part, nblk, 1 + ND, tot);
+  {
+    const double sd = column_sums_all<1 + ND, AB>(part, nblk, red);
+    if ((int)threadIdx.x < 1 + ND) tot[threadIdx.x] = sd;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const double relres = sqrt(tot[0]) * S->nb_inv;

@@ -179,6 +179,21 @@ __device__ inline double block_sums(const double* v, double* red) {
   return s;
 }
 
+// The same sums for a compile-time column count by a 1-D block of NT
+// threads: thread t sums rows t, t + NT, ... of every column (all loads of a
+// thread independent), then block_sums; thread d < NDC returns column d.
+template <int NDC, int NT>
+__device__ inline double column_sums_all(const double* __restrict__ part, int nblk, double* red) {
+  double v[NDC];
+#pragma unroll
+  for (int d = 0; d < NDC; ++d) v[d] = 0;
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+#pragma unroll
+    for (int d = 0; d < NDC; ++d) v[d] += part[(int64_t)d * nblk + b];
+  }
+  return block_sums<NDC, NT>(v, red);
+}
+
 // Fixed-order sums of nd columns of part[nd][nblk] (column-major partials)
 // into tot[nd] by a 1-D block: wave w sums columns w, w + nwaves, ...; the
 // caller synchronises before reading tot.
