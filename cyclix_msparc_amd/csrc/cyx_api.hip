@@ -417,22 +417,27 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   const bool cx = dtype_complex(dtype);
   const size_t esz = cx ? 16 : 8;
   const size_t mat = (size_t)ns * ns * esz;
-  const size_t need = 2 * mat + 4 * (size_t)ns * sizeof(double) + 256;
+  const size_t need = 3 * mat + 4 * (size_t)ns * sizeof(double) + (size_t)ns * esz + 512;
   int rc;
   if ((rc = ensure(h->sub, need)) != CYX_OK) return rc;
   char* base = (char*)h->sub.p;
   void* M = base;
   void* Hs = base + mat;
-  double* D = (double*)(base + 2 * mat);
+  void* X = base + 2 * mat;  // eigenvectors when the reduction runs in one workgroup
+  double* D = (double*)(base + 3 * mat);
   double* E = D + ns;
   double* scal = E + ns;
-  rocblas_int* info = (rocblas_int*)(scal + ns);
+  void* tau = (void*)(base + 3 * mat + (((size_t)3 * ns * sizeof(double) + 15) & ~(size_t)15));
+  rocblas_int* info = (rocblas_int*)((char*)tau + (size_t)ns * esz + 64);
+  // one-workgroup tridiagonal reduction when its LDS fits, else rocSOLVER's heevd
+  const bool own_trd = hetrd_lds_bytes(ns, cx) <= 64 * 1024;
   if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
   if ((rc = stencil_step(h, dtype, psi, nullptr, other, ns, N, 1.0, 0.0, 0.0)) != CYX_OK) return rc;
   // Hs in full: its two triangles are averaged below, as src/eigSolver.m:108 does
   if ((rc = gram_rect(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, Hs, ns)) != CYX_OK) return rc;
+  void* Q = Hs;  // eigenvectors of the pencil
   {
     // Hermitian-definite pencil (Hs, M) in FP64: M = U^H U, Hs' = U^-H Hs U^-1,
     // Hs' = Q' D Q'^H, Q = U^-1 Q' (Q^H M Q = I).
@@ -448,9 +453,21 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
       bs[2] = rocblas_ztrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
-      bs[3] = rocsolver_zheevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      if (own_trd) {
+        // A = Q T Q^H (one workgroup), T = Z diag(D) Z^T, eigenvectors Q Z
+        auto* x = (rocblas_double_complex*)X;
+        if ((rc = hetrd(st, true, a, ns, D, E, tau)) != CYX_OK) return rc;
+        bs[3] = rocsolver_zstedc(bh, rocblas_evect_tridiagonal, ns, D, E, x, ns, info + 1);
+        if (bs[3] == rocblas_status_success)
+          bs[3] = rocsolver_zunmtr(bh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, ns, ns, a, ns,
+                                   (rocblas_double_complex*)tau, x, ns);
+        a = x;
+      } else {
+        bs[3] = rocsolver_zheevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      }
       bs[4] = rocblas_ztrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      Q = a;
     } else {
       const double one = 1.0;
       auto* m = (double*)M;
@@ -460,9 +477,20 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
       bs[2] = rocblas_dtrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
-      bs[3] = rocsolver_dsyevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      if (own_trd) {
+        auto* x = (double*)X;
+        if ((rc = hetrd(st, false, a, ns, D, E, tau)) != CYX_OK) return rc;
+        bs[3] = rocsolver_dstedc(bh, rocblas_evect_tridiagonal, ns, D, E, x, ns, info + 1);
+        if (bs[3] == rocblas_status_success)
+          bs[3] = rocsolver_dormtr(bh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, ns, ns, a, ns,
+                                   (double*)tau, x, ns);
+        a = x;
+      } else {
+        bs[3] = rocsolver_dsyevd(bh, rocblas_evect_original, rocblas_fill_upper, ns, a, ns, D, E, info + 1);
+      }
       bs[4] = rocblas_dtrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
+      Q = a;
     }
     for (int k = 0; k < 5; ++k)
       if (bs[k] != rocblas_status_success) {
@@ -470,7 +498,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
         return CYX_ERR_SOLVER;
       }
   }
-  if ((rc = rotate(h, dtype, psi, Hs, other, ns, N)) != CYX_OK) return rc;
+  if ((rc = rotate(h, dtype, psi, Q, other, ns, N)) != CYX_OK) return rc;
   if ((rc = wnormalize(h, dtype, other, ns, N, scal)) != CYX_OK) return rc;
   rocblas_int hinfo[2] = {0, 0};
   CYX_HIP(hipMemcpyAsync(eigval, D, ns * sizeof(double), hipMemcpyDeviceToHost, st));

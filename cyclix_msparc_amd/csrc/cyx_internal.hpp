@@ -157,6 +157,12 @@ int unet_precondition(cyx_unet_s* net, const double* coef, const double* r, floa
 bool unet_matches(const cyx_unet_s* net, const cyx_grid_desc& g);
 cyx_ctx_s* unet_context(const cyx_unet_s* net);
 
+// Householder reduction of a Hermitian (cx) or symmetric FP64 n x n matrix to
+// real tridiagonal form in one workgroup (LAPACK lower storage of the
+// reflectors); hetrd_lds_bytes(n) <= 64 KiB is required
+size_t hetrd_lds_bytes(int n, bool cx);
+int hetrd(hipStream_t st, bool cx, void* A, int n, double* d, double* e, void* tau);
+
 // Poisson kernels
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
                 double omega, double beta, int m, int p, cyx_aar_info* info);
