@@ -239,6 +239,27 @@ static int upload_kb(KbTables<T>& D, const cyx_projector_desc* d, const double* 
   if ((rc = upload_vec(D.s_il, K.s_il))) return rc;
   if ((rc = upload_vec(D.il_phase, K.il_phase))) return rc;
   if ((rc = upload_vec(D.il_proj, K.il_proj))) return rc;
+  {
+    std::vector<int> nodes;
+    for (int64_t n = 0; n < (int64_t)K.s_pk.size(); ++n)
+      if (K.s_pk[n] & 63u) nodes.push_back((int)n);
+    D.nkbn = (int)nodes.size();
+    if ((rc = upload_vec(D.kb_nodes, nodes))) return rc;
+    std::vector<T> cph(K.s_chi.size());
+    std::vector<int> pj(K.s_chi.size());
+    for (size_t e = 0; e < cph.size(); ++e) {
+      const T ph = K.il_phase[K.s_il[e]];
+      if constexpr (traits<T>::is_complex) {
+        cph[e].x = K.s_chi[e] * ph.x;
+        cph[e].y = K.s_chi[e] * ph.y;
+      } else {
+        cph[e] = K.s_chi[e] * ph;
+      }
+      pj[e] = K.il_proj[K.s_il[e]];
+    }
+    if ((rc = upload_vec(D.s_cph, cph))) return rc;
+    if ((rc = upload_vec(D.s_pj, pj))) return rc;
+  }
   return CYX_OK;
 }
 
@@ -253,6 +274,9 @@ static void free_kb(KbTables<T>& D) {
   (void)hipFree(D.s_il);
   (void)hipFree(D.il_phase);
   (void)hipFree(D.il_proj);
+  (void)hipFree(D.kb_nodes);
+  (void)hipFree(D.s_cph);
+  (void)hipFree(D.s_pj);
   D = KbTables<T>{};
 }
 

@@ -42,6 +42,10 @@ struct KbTables {
   int nimgl = 0;
   T* il_phase = nullptr;  // [nimgl] e^{-i k.d_img}
   int* il_proj = nullptr; // [nimgl]
+  int* kb_nodes = nullptr;  // [nkbn] the nodes with at least one entry, ascending
+  int nkbn = 0;
+  T* s_cph = nullptr;       // [nent] chi * e^{-i k.d_img}, for the standalone scatter
+  int* s_pj = nullptr;      // [nent] projector of the entry
 };
 
 struct DeviceBuf {

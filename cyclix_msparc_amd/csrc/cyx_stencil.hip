@@ -574,6 +574,52 @@ __global__ void __launch_bounds__(256) kb_gather_kernel(const T* __restrict__ Y,
   if (threadIdx.x == 0) C[(int64_t)s * nproj + p] = rmul(gamma[p], red[0]);
 }
 
+// KB scatter as its own pass, after a stencil step without it:
+// Y[s][n] += alpha sum_e cph_e C[s][proj_e], cph_e = chi_e e^{-i k.d_img(e)},
+// over the nodes with entries (the second half of
+// src/h_nonlocal_vector_mult.m:24-32).  A thread takes one node and KSC
+// columns, so the node's entries are read once for KSC columns; consecutive
+// threads take consecutive nodes of one column; the block's KSC columns of C
+// are staged in LDS.
+constexpr int KSC = 8;
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_scatter_kernel(T* __restrict__ Y, int64_t ld, int ncol,
+                                                         const int* __restrict__ kb_nodes, int nkbn,
+                                                         const uint32_t* __restrict__ s_pk,
+                                                         const T* __restrict__ s_cph, const int* __restrict__ s_pj,
+                                                         const T* __restrict__ C, int nproj,
+                                                         typename traits<T>::real alpha) {
+  extern __shared__ __align__(16) unsigned char ksc_raw[];
+  T* Cs = reinterpret_cast<T*>(ksc_raw);  // [KSC][nproj]
+  const int s0 = blockIdx.y * KSC;
+  for (int e = threadIdx.x; e < KSC * nproj; e += 256) {
+    const int c = e / nproj, p = e % nproj;
+    Cs[e] = s0 + c < ncol ? C[(int64_t)(s0 + c) * nproj + p] : zero<T>();
+  }
+  __syncthreads();
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nkbn) return;
+  const int node = kb_nodes[q];
+  const uint32_t pk = s_pk[node];
+  const int h0 = (int)(pk >> 6), cnt = (int)(pk & 63u);
+  T acc[KSC];
+#pragma unroll
+  for (int c = 0; c < KSC; ++c) acc[c] = zero<T>();
+  for (int e = 0; e < cnt; ++e) {
+    const T cph = s_cph[h0 + e];
+    const int p = s_pj[h0 + e];
+#pragma unroll
+    for (int c = 0; c < KSC; ++c) acc[c] = cfma(cph, Cs[c * nproj + p], acc[c]);
+  }
+  T yv[KSC];
+#pragma unroll
+  for (int c = 0; c < KSC; ++c) yv[c] = s0 + c < ncol ? Y[(int64_t)(s0 + c) * ld + node] : zero<T>();
+#pragma unroll
+  for (int c = 0; c < KSC; ++c)
+    if (s0 + c < ncol) Y[(int64_t)(s0 + c) * ld + node] = rfma(alpha, acc[c], yv[c]);
+}
+
 // Batched gather: entry c owns blocks [blk0, blk0 + nproj * ncol)
 template <class T>
 struct GatherEntry {
@@ -787,9 +833,26 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   StepLaunch<T> L;
   if ((rc = make_step<T>(h, in_, prev_, out_, ncol, ld, alpha, shift, beta, 0, h->tune_flags, L)) != CYX_OK)
     return rc;
-  PhaseTimer tm(h->ctx, prev_ ? PH_STENCIL : PH_HAPPLY);
-  launch_single<T>(L, prev_ != nullptr, st);
-  CYX_HIP(hipGetLastError());
+  // the KB scatter runs as its own pass after the stencil: the barrier-
+  // synchronised plane loop then carries no per-node table loads
+  const KbTables<T>& K = kb_of<T>(h);
+  // (systems with so many projectors that the scatter's LDS slice of C would
+  // not fit keep the scatter inside the stencil)
+  const bool kb = L.kb && (size_t)KSC * K.nproj * sizeof(T) <= 32 * 1024;
+  if (kb) L.kb = false;
+  {
+    PhaseTimer tm(h->ctx, prev_ ? PH_STENCIL : PH_HAPPLY);
+    launch_single<T>(L, prev_ != nullptr, st);
+    CYX_HIP(hipGetLastError());
+  }
+  if (kb && K.nkbn > 0) {
+    PhaseTimer tm(h->ctx, PH_KB_GATHER);
+    hipLaunchKernelGGL(kb_scatter_kernel<T>, dim3(ceil_div(K.nkbn, 256), ceil_div(ncol, KSC)), dim3(256),
+                       (size_t)KSC * K.nproj * sizeof(T), st, (T*)out_, ld, ncol, (const int*)K.kb_nodes, K.nkbn,
+                       (const uint32_t*)K.s_pk, (const T*)K.s_cph, (const int*)K.s_pj, (const T*)h->kbC.p, K.nproj,
+                       (typename traits<T>::real)alpha);
+    CYX_HIP(hipGetLastError());
+  }
   return CYX_OK;
 }
 
