@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "cyx_internal.hpp"
+#include "cyx_reduce.hpp"
 
 namespace cyx {
 
@@ -620,6 +621,73 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(T* __restrict__ Y, int6
     if (s0 + c < ncol) Y[(int64_t)(s0 + c) * ld + node] = rfma(alpha, acc[c], yv[c]);
 }
 
+// Two-stage gather for the single-channel step: block (p, chunk, column
+// group) sums g[e] * Y_s[node[e]] over one chunk of projector p's entries for
+// GCOL columns, so each entry's node index and coefficient are loaded once for
+// GCOL columns, and reduces in FP64 (fixed order) into part[s][p][chunk];
+// kb_gather_reduce_kernel sums the chunks in order and applies gamma_p.
+constexpr int GCOL = 8;
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_gather_part_kernel(const T* __restrict__ Y, int64_t ld, int ncol,
+                                                             const int* __restrict__ g_ptr,
+                                                             const int* __restrict__ g_node,
+                                                             const T* __restrict__ g_coef, int nproj, int nchunk,
+                                                             typename acc_of<T>::type* __restrict__ part) {
+  using D = typename acc_of<T>::type;
+  constexpr bool CX = traits<T>::is_complex;
+  constexpr int K = CX ? 2 * GCOL : GCOL;
+  __shared__ double red[(256 / 64) * K];
+  const int p = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+  const int s0 = blockIdx.y * GCOL;
+  const int nc = min(GCOL, ncol - s0);
+  const int e0 = g_ptr[p], ne = g_ptr[p + 1] - e0;
+  const int c0 = e0 + (int)((int64_t)ne * ch / nchunk), c1 = e0 + (int)((int64_t)ne * (ch + 1) / nchunk);
+  T acc[GCOL];
+#pragma unroll
+  for (int c = 0; c < GCOL; ++c) acc[c] = zero<T>();
+  const T* y = Y + (int64_t)s0 * ld;
+  for (int e = c0 + threadIdx.x; e < c1; e += 256) {
+    const int node = g_node[e];
+    const T gc = g_coef[e];
+    T yv[GCOL];
+#pragma unroll
+    for (int c = 0; c < GCOL; ++c) yv[c] = c < nc ? y[(int64_t)c * ld + node] : zero<T>();
+#pragma unroll
+    for (int c = 0; c < GCOL; ++c) acc[c] = cfma(gc, yv[c], acc[c]);
+  }
+  double v[K];
+#pragma unroll
+  for (int c = 0; c < GCOL; ++c) {
+    if constexpr (CX) {
+      v[2 * c] = (double)acc[c].x;
+      v[2 * c + 1] = (double)acc[c].y;
+    } else {
+      v[c] = (double)acc[c];
+    }
+  }
+  const double sum = block_sums<K, 256>(v, red);
+  if ((int)threadIdx.x < K) {
+    const int c = CX ? threadIdx.x / 2 : threadIdx.x;
+    if (c < nc) {
+      double* out = reinterpret_cast<double*>(part + ((int64_t)(s0 + c) * nproj + p) * nchunk + ch);
+      out[CX ? (threadIdx.x & 1) : 0] = sum;
+    }
+  }
+}
+
+template <class T>
+__global__ void kb_gather_reduce_kernel(const typename acc_of<T>::type* __restrict__ part, int ncol, int nproj,
+                                        int nchunk, const typename traits<T>::real* __restrict__ gamma,
+                                        T* __restrict__ C) {
+  using D = typename acc_of<T>::type;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // s * nproj + p
+  if (idx >= ncol * nproj) return;
+  D sum{};
+  for (int ch = 0; ch < nchunk; ++ch) sum = add(sum, part[(int64_t)idx * nchunk + ch]);
+  C[idx] = narrow<T>(rmul((double)gamma[idx % nproj], sum));
+}
+
 // Batched gather: entry c owns blocks [blk0, blk0 + nproj * ncol)
 template <class T>
 struct GatherEntry {
@@ -814,9 +882,20 @@ template <class T>
 static int gather_launch(cyx_ham_s* h, const void* in_, int ncol, int64_t ld, hipStream_t st) {
   const KbTables<T>& K = kb_of<T>(h);
   if (K.nproj == 0) return CYX_OK;
+  using D = typename acc_of<T>::type;
+  // about 2048 blocks: projectors x entry chunks x column groups
+  const int ngrp = ceil_div(ncol, GCOL);
+  const int nchunk = std::max(1, std::min(64, ceil_div(2048, K.nproj * ngrp)));
   CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
-  hipLaunchKernelGGL(kb_gather_kernel<T>, dim3(K.nproj, ncol), dim3(256), 0, st, (const T*)in_, ld, K.g_ptr,
-                     K.g_node, K.g_coef, K.gamma, K.nproj, (T*)h->kbC.p);
+  // the partials share the Gram kernels' scratch (stream-ordered uses; filter
+  // plans keep pointers to kbC, so kbC itself never grows here)
+  CYX_CHECK(ensure(h->small, (size_t)ncol * K.nproj * nchunk * sizeof(D)) == CYX_OK,
+            "kb scratch allocation failed");
+  D* part = (D*)h->small.p;
+  hipLaunchKernelGGL(kb_gather_part_kernel<T>, dim3(K.nproj * nchunk, ngrp), dim3(256), 0, st, (const T*)in_, ld,
+                     ncol, K.g_ptr, K.g_node, K.g_coef, K.nproj, nchunk, part);
+  hipLaunchKernelGGL(kb_gather_reduce_kernel<T>, dim3(ceil_div(ncol * K.nproj, 256)), dim3(256), 0, st,
+                     (const D*)part, ncol, K.nproj, nchunk, K.gamma, (T*)h->kbC.p);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
