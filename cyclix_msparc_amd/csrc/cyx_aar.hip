@@ -102,9 +102,28 @@ __device__ void finalize_richardson(const FinArgs& F) {
 // launched, finalises the previous Richardson iteration.  T is scratch, so
 // the theta blocks run whatever the convergence state.
 __global__ void __launch_bounds__(PB) theta_kernel(PoissonOp P, const double* __restrict__ x, double* __restrict__ T,
-                                                   int ntb, FinArgs F) {
+                                                   int ntb, int pair, FinArgs F) {
   if ((int)blockIdx.x == ntb) {
     finalize_richardson(F);
+    return;
+  }
+  if (pair) {
+    // two consecutive radii per thread, 16-byte loads and stores
+    const int hr = P.nr >> 1;
+    const int pi = blockIdx.x * PB + threadIdx.x;
+    const int row = pi / hr;  // k * nt + j
+    if (row >= P.nz * P.nt) return;
+    const int i = 2 * (pi - row * hr), j = row % P.nt;
+    const int n = row * P.nr + i, base = n - j * P.nr;
+    double ta = 0, tb = 0;
+#pragma unroll
+    for (int p = 1; p <= FDN; ++p) {
+      const double2 u = *reinterpret_cast<const double2*>(x + base + wrapi(j + p, P.nt) * P.nr);
+      const double2 d = *reinterpret_cast<const double2*>(x + base + wrapi(j - p, P.nt) * P.nr);
+      ta = __builtin_fma(P.wt1[p], u.x - d.x, ta);
+      tb = __builtin_fma(P.wt1[p], u.y - d.y, tb);
+    }
+    *reinterpret_cast<double2*>(T + n) = make_double2(ta, tb);
     return;
   }
   const Pt q = point_of(P);
@@ -115,6 +134,16 @@ __global__ void __launch_bounds__(PB) theta_kernel(PoissonOp P, const double* __
   for (int p = 1; p <= FDN; ++p)
     t = __builtin_fma(P.wt1[p], x[base + wrapi(q.j + p, P.nt) * P.nr] - x[base + wrapi(q.j - p, P.nt) * P.nr], t);
   T[q.n] = t;
+}
+
+// Launch shape of theta_kernel: two radii per thread when nr is even and both
+// vectors are 16-byte aligned
+struct ThetaShape {
+  int ntb, pair;
+};
+static ThetaShape theta_shape(const PoissonOp& P, const double* x, const double* T) {
+  const int pair = ((P.nr & 1) == 0 && (((uintptr_t)x | (uintptr_t)T) & 15) == 0) ? 1 : 0;
+  return ThetaShape{(int)ceil_div((int64_t)P.N / (pair ? 2 : 1), PB), pair};
 }
 
 __device__ inline double lap_at(const PoissonOp& P, const double* __restrict__ x, const double* __restrict__ T,
@@ -490,7 +519,10 @@ int poisson_apply(cyx_poisson_s* ps, const double* x, double* y) {
   const int nblk = (int)ceil_div(N, PB);
   CYX_CHECK(ensure(ps->tbuf, (size_t)N * sizeof(double)) == CYX_OK, "allocation failed");
   double* T = (double*)ps->tbuf.p;
-  if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, x, T, nblk, FinArgs{});
+  if (P.helical) {
+    const ThetaShape ts = theta_shape(P, x, T);
+    hipLaunchKernelGGL(theta_kernel, dim3(ts.ntb), dim3(PB), 0, st, P, x, T, ts.ntb, ts.pair, FinArgs{});
+  }
   hipLaunchKernelGGL(apply_kernel, dim3(nblk), dim3(PB), 0, st, P, x, (const double*)T, y);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
@@ -543,7 +575,6 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   hipLaunchKernelGGL(norm_kernel, dim3(nblk_norm), dim3(AB), 0, st, f, N, part);
   hipLaunchKernelGGL(norm_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk_norm, S);
   const PoissonOp P = make_op(ps);
-  const int ntb = P.helical ? nblk : 0;  // theta blocks per launch
   AarState hs{};
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
@@ -576,9 +607,10 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       A.xbuf = xb;
       A.tol = tol;
       A.omega = omega;
-      if (ntb > 0 || fin_pending)
-        hipLaunchKernelGGL(theta_kernel, dim3(ntb + (fin_pending ? 1 : 0)), dim3(PB), 0, st, P, A.xp, T, ntb,
-                           fin_args(count - 1));
+      const ThetaShape ts = P.helical ? theta_shape(P, A.xp, T) : ThetaShape{0, 0};
+      if (ts.ntb > 0 || fin_pending)
+        hipLaunchKernelGGL(theta_kernel, dim3(ts.ntb + (fin_pending ? 1 : 0)), dim3(PB), 0, st, P, A.xp, T, ts.ntb,
+                           ts.pair, fin_args(count - 1));
       fin_pending = false;
       if (!A.anderson) {
         hipLaunchKernelGGL(aar_step_brick_kernel<0>, dim3(nstep), dim3(PB), 0, st, P, A);
@@ -598,16 +630,22 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
           // learned correction of the Anderson iterate, with exact line search
           double* xn = X[xb ^ 1];
           const AarState* cS = S;
-          if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)xn, T, nblk,
-                                            FinArgs{});
+          if (P.helical) {
+            const ThetaShape tn = theta_shape(P, xn, T);
+            hipLaunchKernelGGL(theta_kernel, dim3(tn.ntb), dim3(PB), 0, st, P, (const double*)xn, T, tn.ntb, tn.pair,
+                               FinArgs{});
+          }
           hipLaunchKernelGGL(resid_kernel, dim3(nblk), dim3(PB), 0, st, P, f, (const double*)xn, (const double*)T,
                              r1buf, cS);
           int rc = unet_precondition(net, ps->coef, r1buf, ubuf);
           if (rc != CYX_OK) return rc;
           hipLaunchKernelGGL(ls_dir_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)r1buf, (const float*)ubuf,
                              dbuf, cS);
-          if (P.helical) hipLaunchKernelGGL(theta_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)dbuf, T,
-                                            nblk, FinArgs{});
+          if (P.helical) {
+            const ThetaShape tn = theta_shape(P, dbuf, T);
+            hipLaunchKernelGGL(theta_kernel, dim3(tn.ntb), dim3(PB), 0, st, P, (const double*)dbuf, T, tn.ntb,
+                               tn.pair, FinArgs{});
+          }
           hipLaunchKernelGGL(ls_apply_kernel, dim3(nblk), dim3(PB), 0, st, P, (const double*)dbuf, (const double*)T,
                              (const double*)r1buf, part, cS);
           hipLaunchKernelGGL(ls_finalize_kernel, dim3(1), dim3(AB), 0, st, (const double*)part, nblk, S);
@@ -616,7 +654,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       }
     }
     if (fin_pending) {
-      hipLaunchKernelGGL(theta_kernel, dim3(1), dim3(PB), 0, st, P, (const double*)X[(count - 1) & 1], T, 0,
+      hipLaunchKernelGGL(theta_kernel, dim3(1), dim3(PB), 0, st, P, (const double*)X[(count - 1) & 1], T, 0, 0,
                          fin_args(count - 1));
       fin_pending = false;
     }
