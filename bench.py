@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per recurrence launch of the fused stencil at this workload, from
 # the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
 # calibration) recorded in profiles/scf_step_cnt10_helical.md
-MEASURED_TRAFFIC = {("cnt10_helical", 16): 4.2697e+09, ("cnt10_helical", 8): 2.1971e+09}
+MEASURED_TRAFFIC = {("cnt10_helical", 16): 3.6669e+09, ("cnt10_helical", 8): 1.8485e+09}
 METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM peak"
 
 
@@ -113,7 +113,7 @@ def run_path(dev, sysm, args, precision, unet=None):
     tm = dev.timing()
     dev.set_timing(False)
     it.close()
-    # roofline of the dominant kernel, the fused stencil + three-term step:
+    # roofline of the dominant kernel, the fused local stencil + three-term step:
     # algorithmic bytes per launch = read Y + read X_prev + write Y_new (ns
     # columns of N points, bsz bytes) + Veff (N doubles, shared by columns)
     st_ms, st_n = tm["stencil"]
@@ -133,7 +133,7 @@ def run_path(dev, sysm, args, precision, unet=None):
         "dtype": "complex128" if precision == "fp64" else "complex64 (FP64 accumulation, FP64 dense eig/energies)",
         "learned_poisson": unet is not None,
         "roofline": {"bound": "hbm", "kernel": f"stencil_step_kernel<{'c128' if precision == 'fp64' else 'c64'}> "
-                                               "(fused H-apply + Chebyshev step)",
+                                               "(fused local H-apply + Chebyshev step; the KB term is its own gather/scatter launches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((args.workload, bsz)),
