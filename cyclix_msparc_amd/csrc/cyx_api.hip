@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -441,7 +442,16 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   const bool cx = dtype_complex(dtype);
   const size_t esz = cx ? 16 : 8;
   const size_t mat = (size_t)ns * ns * esz;
-  const size_t need = 3 * mat + 4 * (size_t)ns * sizeof(double) + (size_t)ns * esz + 512;
+  // tridiagonal reduction: distributed over workgroups with the matrix in LDS,
+  // else one workgroup, else rocSOLVER's heevd (CYX_TRIDIAG = dist | one |
+  // lapack forces a path where it fits; read per call)
+  const char* trd_env = getenv("CYX_TRIDIAG");
+  const std::string trd_req = trd_env ? trd_env : "";
+  const bool dist_fits = hetrd_dist_groups(ns, cx) > 0, one_fits = hetrd_lds_bytes(ns, cx) <= 64 * 1024;
+  const bool use_dist = dist_fits && (trd_req.empty() || trd_req == "dist");
+  const bool own_trd = use_dist || (one_fits && trd_req != "lapack");
+  const size_t dist_ws = use_dist ? hetrd_dist_ws_bytes(ns, cx) : 0;
+  const size_t need = 3 * mat + 4 * (size_t)ns * sizeof(double) + (size_t)ns * esz + 512 + dist_ws + 256;
   int rc;
   if ((rc = ensure(h->sub, need)) != CYX_OK) return rc;
   char* base = (char*)h->sub.p;
@@ -453,8 +463,11 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   double* scal = E + ns;
   void* tau = (void*)(base + 3 * mat + (((size_t)3 * ns * sizeof(double) + 15) & ~(size_t)15));
   rocblas_int* info = (rocblas_int*)((char*)tau + (size_t)ns * esz + 64);
-  // one-workgroup tridiagonal reduction when its LDS fits, else rocSOLVER's heevd
-  const bool own_trd = hetrd_lds_bytes(ns, cx) <= 64 * 1024;
+  void* trd_ws = (void*)(((uintptr_t)(info + 16) + 255) & ~(uintptr_t)255);
+  unsigned* trd_flag = nullptr;
+  auto reduce = [&](void* a) -> int {
+    return use_dist ? hetrd_dist(st, cx, a, ns, D, E, tau, trd_ws, &trd_flag) : hetrd(st, cx, a, ns, D, E, tau);
+  };
   if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
   if ((rc = stencil_step(h, dtype, psi, nullptr, other, ns, N, 1.0, 0.0, 0.0)) != CYX_OK) return rc;
   // Hs in full: its two triangles are averaged below, as src/eigSolver.m:108 does
@@ -480,7 +493,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
       if (own_trd) {
         // A = Q T Q^H (one workgroup), T = Z diag(D) Z^T, eigenvectors Q Z
         auto* x = (rocblas_double_complex*)X;
-        if ((rc = hetrd(st, true, a, ns, D, E, tau)) != CYX_OK) return rc;
+        if ((rc = reduce(a)) != CYX_OK) return rc;
         bs[3] = rocsolver_zstedc(bh, rocblas_evect_tridiagonal, ns, D, E, x, ns, info + 1);
         if (bs[3] == rocblas_status_success)
           bs[3] = rocsolver_zunmtr(bh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, ns, ns, a, ns,
@@ -503,7 +516,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
       if (own_trd) {
         auto* x = (double*)X;
-        if ((rc = hetrd(st, false, a, ns, D, E, tau)) != CYX_OK) return rc;
+        if ((rc = reduce(a)) != CYX_OK) return rc;
         bs[3] = rocsolver_dstedc(bh, rocblas_evect_tridiagonal, ns, D, E, x, ns, info + 1);
         if (bs[3] == rocblas_status_success)
           bs[3] = rocsolver_dormtr(bh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, ns, ns, a, ns,
@@ -525,9 +538,15 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   if ((rc = rotate(h, dtype, psi, Q, other, ns, N)) != CYX_OK) return rc;
   if ((rc = wnormalize(h, dtype, other, ns, N, scal)) != CYX_OK) return rc;
   rocblas_int hinfo[2] = {0, 0};
+  unsigned trd_failed = 0;
   CYX_HIP(hipMemcpyAsync(eigval, D, ns * sizeof(double), hipMemcpyDeviceToHost, st));
   CYX_HIP(hipMemcpyAsync(hinfo, info, 2 * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  if (trd_flag) CYX_HIP(hipMemcpyAsync(&trd_failed, trd_flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   CYX_HIP(hipStreamSynchronize(st));
+  if (trd_failed != 0) {
+    set_error("tridiagonal reduction: a workgroup barrier timed out (workgroups not co-resident?)");
+    return CYX_ERR_SOLVER;
+  }
   if (hinfo[0] != 0) {
     set_error("overlap matrix not positive definite (potrf info = %d): the filtered block is rank deficient",
               (int)hinfo[0]);

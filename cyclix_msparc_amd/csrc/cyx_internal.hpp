@@ -166,6 +166,14 @@ cyx_ctx_s* unet_context(const cyx_unet_s* net);
 // reflectors); hetrd_lds_bytes(n) <= 64 KiB is required
 size_t hetrd_lds_bytes(int n, bool cx);
 int hetrd(hipStream_t st, bool cx, void* A, int n, double* d, double* e, void* tau);
+// The same reduction over up to 32 workgroups with the matrix in LDS (one
+// grid barrier per column); 0 groups: n does not fit.  ws holds
+// hetrd_dist_ws_bytes(n) bytes; *flag points at a device word that is
+// nonzero after the launch if a barrier wait gave up.
+int hetrd_dist_groups(int n, bool cx);
+size_t hetrd_dist_ws_bytes(int n, bool cx);
+int hetrd_dist(hipStream_t st, bool cx, void* A, int n, double* d, double* e, void* tau, void* ws,
+               unsigned** flag);
 
 // Poisson kernels
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,

@@ -163,6 +163,37 @@ def test_rayleigh_ritz_many_states(device, name, dt):
     ham.close()
 
 
+@pytest.mark.parametrize("name,dt,ns", [("cyclic_gamma", "f64", 150), ("cyclic_gamma", "f64", 256),
+                                        ("helical_bloch", "c128", 150), ("helical_bloch", "c128", 256)])
+def test_rayleigh_ritz_tridiagonal_paths(device, name, dt, ns, monkeypatch):
+    """The dense pencil's tridiagonal reduction three ways on the same block:
+    over several workgroups with the matrix in LDS (2, 4 groups for the real
+    sizes; 3, 8 for the complex ones), in one workgroup, and by rocSOLVER's
+    heevd.  Each matches the oracle's metric Rayleigh-Ritz
+    (src/eigSolver.m:99-124) in eigenvalues and spanned subspace."""
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    N = sysm.grid.N
+    X = rand_block(N, ns, dt, 23)
+    ev_ref, psi_ref = O.rayleigh_ritz_w(oham, X)
+    Wm = O.metric_weights(oham.g)
+    V = np.random.default_rng(6).standard_normal((N, 4))
+    P_ref = _subspace_projector(psi_ref, Wm, V)
+    for mode in ("dist", "one", "lapack"):
+        monkeypatch.setenv("CYX_TRIDIAG", mode)
+        psi_d = to_dev(X, dt, device)
+        ev = ham.rayleigh_ritz(psi_d)
+        psi = to_host(psi_d)
+        err = np.max(np.abs(ev - ev_ref)) / np.max(np.abs(ev_ref))
+        print(f"rayleigh-ritz {name} {dt} ns={ns} tridiagonal={mode}: eigenvalue rel err {err:.3e}", flush=True)
+        assert err < 1e-9
+        assert rel(_subspace_projector(psi, Wm, V), P_ref) < 1e-8
+        nrm = np.sum(sysm.W[:, None] * np.abs(psi) ** 2, axis=0)
+        assert np.allclose(nrm, 1.0, atol=1e-12)
+    ham.close()
+
+
 @pytest.mark.parametrize("kpt,twist", [((0, 0, 0), 0.0), ((0, 1.0, 0.6), 0.05)])
 def test_chefsi_converges_to_dense_eigenvalues(device, kpt, twist):
     """Repeated device passes reach the lowest eigenvalues of the dense H."""
