@@ -131,7 +131,7 @@ static int upload_coef(OpCoef<R>& C, const cyx_grid_desc& g, const std::vector<d
 template <class T>
 struct HostKb {
   using R = typename traits<T>::real;
-  std::vector<int> g_ptr, g_node, s_ptr, s_il, il_proj;
+  std::vector<int> g_ptr, g_node, s_ptr, s_il, il_proj, pr_lo, pr_hi, blk_pair, pp_ptr;
   std::vector<uint32_t> s_pk;
   int max_count = 0;
   std::vector<T> g_coef, il_phase;
@@ -183,6 +183,11 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
     off += nn;
     choff += (int64_t)nn * d->angnum[a];
   }
+  // each projector's entries ascend by node (images interleave), so that a
+  // node range is one contiguous entry range per projector
+  for (int p = 0; p < nproj; ++p)
+    std::stable_sort(byproj[p].begin(), byproj[p].end(),
+                     [](const Ent& x, const Ent& y) { return x.node != y.node ? x.node < y.node : x.il < y.il; });
   K.g_ptr.assign(nproj + 1, 0);
   for (int p = 0; p < nproj; ++p) {
     K.g_ptr[p + 1] = K.g_ptr[p] + (int)byproj[p].size();
@@ -209,6 +214,46 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
     K.max_count = std::max(K.max_count, c);
     K.s_pk[n] = ((uint32_t)K.s_ptr[n] << 6) | (uint32_t)std::min(c, 63);
   }
+  // gather chunks: node ranges holding about KB_CHUNK_ENTRIES entries of all
+  // projectors together (cnt[n + 1] is node n's entry count)
+  constexpr int KB_CHUNK_ENTRIES = 4096;
+  std::vector<int64_t> bound{0};
+  for (int64_t n = 0, acc = 0; n < N; ++n) {
+    acc += cnt[n + 1];
+    if (acc >= KB_CHUNK_ENTRIES && n + 1 < N) {
+      bound.push_back(n + 1);
+      acc = 0;
+    }
+  }
+  bound.push_back(N);
+  const int nch = (int)bound.size() - 1;
+  // pair ids run projector-major, in chunk order within a projector
+  std::vector<std::vector<int>> by_chunk(nch);  // pair ids of each chunk, by projector
+  K.pp_ptr.assign(nproj + 1, 0);
+  for (int p = 0; p < nproj; ++p) {
+    const int e0 = K.g_ptr[p], e1 = K.g_ptr[p + 1];
+    int e = e0;
+    for (int c = 0; c < nch && e < e1; ++c) {
+      const int lo = e;
+      while (e < e1 && K.g_node[e] < bound[c + 1]) ++e;
+      if (e > lo) {
+        const int q = (int)K.pr_lo.size();
+        K.pr_lo.push_back(lo);
+        K.pr_hi.push_back(e);
+        by_chunk[c].push_back(q);
+      }
+    }
+    K.pp_ptr[p + 1] = (int)K.pr_lo.size();
+  }
+  // pairs of chunk c go to launch slots congruent to c mod 8 (one XCD)
+  std::vector<std::vector<int>> slot(8);
+  for (int c = 0; c < nch; ++c)
+    for (int q : by_chunk[c]) slot[c % 8].push_back(q);
+  size_t len = 0;
+  for (const auto& v : slot) len = std::max(len, v.size());
+  K.blk_pair.assign(8 * len, -1);
+  for (int x = 0; x < 8; ++x)
+    for (size_t i = 0; i < slot[x].size(); ++i) K.blk_pair[8 * i + x] = slot[x][i];
 }
 
 template <class V>
@@ -240,6 +285,12 @@ static int upload_kb(KbTables<T>& D, const cyx_projector_desc* d, const double* 
   if ((rc = upload_vec(D.s_il, K.s_il))) return rc;
   if ((rc = upload_vec(D.il_phase, K.il_phase))) return rc;
   if ((rc = upload_vec(D.il_proj, K.il_proj))) return rc;
+  D.npair = (int)K.pr_lo.size();
+  D.nblk_pair = (int)K.blk_pair.size();
+  if ((rc = upload_vec(D.pr_lo, K.pr_lo))) return rc;
+  if ((rc = upload_vec(D.pr_hi, K.pr_hi))) return rc;
+  if ((rc = upload_vec(D.blk_pair, K.blk_pair))) return rc;
+  if ((rc = upload_vec(D.pp_ptr, K.pp_ptr))) return rc;
   {
     std::vector<int> nodes;
     for (int64_t n = 0; n < (int64_t)K.s_pk.size(); ++n)
@@ -278,6 +329,10 @@ static void free_kb(KbTables<T>& D) {
   (void)hipFree(D.kb_nodes);
   (void)hipFree(D.s_cph);
   (void)hipFree(D.s_pj);
+  (void)hipFree(D.pr_lo);
+  (void)hipFree(D.pr_hi);
+  (void)hipFree(D.blk_pair);
+  (void)hipFree(D.pp_ptr);
   D = KbTables<T>{};
 }
 

@@ -46,6 +46,17 @@ struct KbTables {
   int nkbn = 0;
   T* s_cph = nullptr;       // [nent] chi * e^{-i k.d_img}, for the standalone scatter
   int* s_pj = nullptr;      // [nent] projector of the entry
+  // two-stage gather over node-range chunks shared by all projectors (each
+  // projector's entries ascend by node): pair q = (projector, chunk) covers
+  // that projector's entries [pr_lo[q], pr_hi[q]) in the chunk; block x takes
+  // pair blk_pair[x] (-1: none), so that the pairs of one chunk run on one
+  // XCD back to back; pairs are numbered projector-major in chunk order, so
+  // projector p owns pairs [pp_ptr[p], pp_ptr[p+1])
+  int npair = 0, nblk_pair = 0;
+  int* pr_lo = nullptr;     // [npair]
+  int* pr_hi = nullptr;     // [npair]
+  int* blk_pair = nullptr;  // [nblk_pair], a multiple of 8
+  int* pp_ptr = nullptr;    // [nproj+1]
 };
 
 struct DeviceBuf {

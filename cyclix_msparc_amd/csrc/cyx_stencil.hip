@@ -621,28 +621,34 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(T* __restrict__ Y, int6
     if (s0 + c < ncol) Y[(int64_t)(s0 + c) * ld + node] = rfma(alpha, acc[c], yv[c]);
 }
 
-// Two-stage gather for the single-channel step: block (p, chunk, column
-// group) sums g[e] * Y_s[node[e]] over one chunk of projector p's entries for
-// GCOL columns, so each entry's node index and coefficient are loaded once for
-// GCOL columns, and reduces in FP64 (fixed order) into part[s][p][chunk];
-// kb_gather_reduce_kernel sums the chunks in order and applies gamma_p.
+// Two-stage gather for the single-channel step.  The node range is cut into
+// chunks shared by all projectors; block (pair, column group) sums
+// g[e] * Y_s[node[e]] over one projector's entries in one chunk for GCOL
+// columns, so each entry's node index and coefficient are loaded once for
+// GCOL columns, and reduces in FP64 (fixed order) into part[s][pair].  The
+// pairs of one chunk run back to back on one XCD (the launch order comes
+// from the host tables), so the Y values that several projectors' footprints
+// share are read from HBM once and then from that XCD's L2.
+// kb_gather_reduce_kernel sums each projector's pairs (numbered projector-
+// major, chunk order) and applies gamma_p.
 constexpr int GCOL = 8;
 
 template <class T>
 __global__ void __launch_bounds__(256) kb_gather_part_kernel(const T* __restrict__ Y, int64_t ld, int ncol,
-                                                             const int* __restrict__ g_ptr,
+                                                             const int* __restrict__ blk_pair,
+                                                             const int* __restrict__ pr_lo,
+                                                             const int* __restrict__ pr_hi,
                                                              const int* __restrict__ g_node,
-                                                             const T* __restrict__ g_coef, int nproj, int nchunk,
+                                                             const T* __restrict__ g_coef, int npair,
                                                              typename acc_of<T>::type* __restrict__ part) {
-  using D = typename acc_of<T>::type;
   constexpr bool CX = traits<T>::is_complex;
   constexpr int K = CX ? 2 * GCOL : GCOL;
   __shared__ double red[(256 / 64) * K];
-  const int p = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+  const int q = blk_pair[blockIdx.x];
+  if (q < 0) return;  // padding of the XCD-ordered launch (whole block)
   const int s0 = blockIdx.y * GCOL;
   const int nc = min(GCOL, ncol - s0);
-  const int e0 = g_ptr[p], ne = g_ptr[p + 1] - e0;
-  const int c0 = e0 + (int)((int64_t)ne * ch / nchunk), c1 = e0 + (int)((int64_t)ne * (ch + 1) / nchunk);
+  const int c0 = pr_lo[q], c1 = pr_hi[q];
   T acc[GCOL];
 #pragma unroll
   for (int c = 0; c < GCOL; ++c) acc[c] = zero<T>();
@@ -670,22 +676,36 @@ __global__ void __launch_bounds__(256) kb_gather_part_kernel(const T* __restrict
   if ((int)threadIdx.x < K) {
     const int c = CX ? threadIdx.x / 2 : threadIdx.x;
     if (c < nc) {
-      double* out = reinterpret_cast<double*>(part + ((int64_t)(s0 + c) * nproj + p) * nchunk + ch);
+      double* out = reinterpret_cast<double*>(part + (int64_t)(s0 + c) * npair + q);
       out[CX ? (threadIdx.x & 1) : 0] = sum;
     }
   }
 }
 
+// one wavefront per (column, projector): lane k sums pairs k, k + 64, ...
+// of the projector (contiguous, chunk order), then a fixed butterfly
 template <class T>
-__global__ void kb_gather_reduce_kernel(const typename acc_of<T>::type* __restrict__ part, int ncol, int nproj,
-                                        int nchunk, const typename traits<T>::real* __restrict__ gamma,
-                                        T* __restrict__ C) {
+__global__ void __launch_bounds__(256) kb_gather_reduce_kernel(const typename acc_of<T>::type* __restrict__ part,
+                                                               int ncol, int nproj, int npair,
+                                                               const int* __restrict__ pp_ptr,
+                                                               const typename traits<T>::real* __restrict__ gamma,
+                                                               T* __restrict__ C) {
   using D = typename acc_of<T>::type;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // s * nproj + p
-  if (idx >= ncol * nproj) return;
-  D sum{};
-  for (int ch = 0; ch < nchunk; ++ch) sum = add(sum, part[(int64_t)idx * nchunk + ch]);
-  C[idx] = narrow<T>(rmul((double)gamma[idx % nproj], sum));
+  const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);  // s * nproj + p
+  const int lane = threadIdx.x & 63;
+  if (idx >= ncol * nproj) return;  // whole wavefronts
+  const int s = idx / nproj, p = idx - s * nproj;
+  const D* ps = part + (int64_t)s * npair + pp_ptr[p];
+  const int n = pp_ptr[p + 1] - pp_ptr[p];
+  D acc{};
+  for (int k = lane; k < n; k += 64) acc = add(acc, ps[k]);
+  if constexpr (traits<T>::is_complex) {
+    const double re = wave_sum(acc.x), im = wave_sum(acc.y);
+    if (lane == 0) C[idx] = narrow<T>(rmul((double)gamma[p], D{re, im}));
+  } else {
+    const double v = wave_sum(acc);
+    if (lane == 0) C[idx] = narrow<T>(rmul((double)gamma[p], D(v)));
+  }
 }
 
 // Batched gather: entry c owns blocks [blk0, blk0 + nproj * ncol)
@@ -883,19 +903,18 @@ static int gather_launch(cyx_ham_s* h, const void* in_, int ncol, int64_t ld, hi
   const KbTables<T>& K = kb_of<T>(h);
   if (K.nproj == 0) return CYX_OK;
   using D = typename acc_of<T>::type;
-  // about 2048 blocks: projectors x entry chunks x column groups
   const int ngrp = ceil_div(ncol, GCOL);
-  const int nchunk = std::max(1, std::min(64, ceil_div(2048, K.nproj * ngrp)));
   CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
   // the partials share the Gram kernels' scratch (stream-ordered uses; filter
   // plans keep pointers to kbC, so kbC itself never grows here)
-  CYX_CHECK(ensure(h->small, (size_t)ncol * K.nproj * nchunk * sizeof(D)) == CYX_OK,
+  CYX_CHECK(ensure(h->small, (size_t)ncol * std::max(K.npair, 1) * sizeof(D)) == CYX_OK,
             "kb scratch allocation failed");
   D* part = (D*)h->small.p;
-  hipLaunchKernelGGL(kb_gather_part_kernel<T>, dim3(K.nproj * nchunk, ngrp), dim3(256), 0, st, (const T*)in_, ld,
-                     ncol, K.g_ptr, K.g_node, K.g_coef, K.nproj, nchunk, part);
-  hipLaunchKernelGGL(kb_gather_reduce_kernel<T>, dim3(ceil_div(ncol * K.nproj, 256)), dim3(256), 0, st,
-                     (const D*)part, ncol, K.nproj, nchunk, K.gamma, (T*)h->kbC.p);
+  if (K.npair > 0)
+    hipLaunchKernelGGL(kb_gather_part_kernel<T>, dim3(K.nblk_pair, ngrp), dim3(256), 0, st, (const T*)in_, ld, ncol,
+                       K.blk_pair, K.pr_lo, K.pr_hi, K.g_node, K.g_coef, K.npair, part);
+  hipLaunchKernelGGL(kb_gather_reduce_kernel<T>, dim3(ceil_div(ncol * K.nproj, 4)), dim3(256), 0, st,
+                     (const D*)part, ncol, K.nproj, K.npair, K.pp_ptr, K.gamma, (T*)h->kbC.p);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
