@@ -384,7 +384,14 @@ int hetrd_dist(hipStream_t st, bool cx, void* A, int n, double* d, double* e, vo
   const void* fn = cx ? (const void*)&hetrd_dist_kernel<cplx<double>> : (const void*)&hetrd_dist_kernel<double>;
   CYX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   void* args[] = {&A, &n, (void*)&G, (void*)&cpw, &d, &e, &tau, &pbuf, &cbuf, &sync};
-  CYX_HIP(hipLaunchCooperativeKernel(fn, dim3(G), dim3(TRD2_NT), args, (unsigned)shm, st));
+  if (hipLaunchCooperativeKernel(fn, dim3(G), dim3(TRD2_NT), args, (unsigned)shm, st) != hipSuccess) {
+    // the runtime refused a co-resident launch: the one-workgroup reduction
+    // gives the same factorisation (its failure word stays clear)
+    (void)hipGetLastError();
+    if (hetrd_lds_bytes(n, cx) <= 64 * 1024) return hetrd(st, cx, A, n, d, e, tau);
+    set_error("tridiagonal reduction of n = %d: cooperative launch refused", n);
+    return CYX_ERR_HIP;
+  }
   return CYX_OK;
 }
 
