@@ -248,13 +248,14 @@ __global__ void __launch_bounds__(AB) norm_finalize_kernel(const double* part, i
 struct StepArgs {
   const double* f;
   const double* xp;     // x_prev (its residual is measured)
-  double* xo;           // Xold on entry; x_new on exit of a Richardson step
-  const double* T;      // theta derivative of x_prev
+  const double* xold;   // the iterate before x_prev (history column)
+  double* xnew;         // x_new of a Richardson step
   double* res;          // preconditioned residual of x_prev (out)
   const double* fold;   // previous preconditioned residual
   double* DX;
   double* DF;
-  double* part;         // [nd][gridDim.x]
+  double* part;         // [nd][nstep]
+  int nstep;            // step blocks (the partials' stride)
   AarState* S;
   int hcol;             // history column written this iteration (-1: none)
   int anderson;         // 1 on Anderson steps
@@ -269,9 +270,13 @@ struct StepArgs {
 // consecutive points of a plane in the r-fastest order) of planes
 // 8 zc .. 8 zc + 7, wave w one plane.  LDS holds x at planes k - FDN ..
 // k + FDN of the block's planes over the segment and an r halo of FDN on
-// each side, T at the same planes over the segment, and the 14 coefficients
+// each side, the helical theta derivative T = (-2 tau D_theta) x at the same
+// planes over the segment (computed here from x), and the 14 coefficients
 // of the segment's radii.  Halo positions outside the plane are clamped into
-// it: they only meet radial taps whose coefficient is zero.
+// it: they only meet radial taps whose coefficient is zero.  With fin set,
+// one block more (the last) finalises the previous Richardson iteration; the
+// iterates are triple-buffered, so this step's stores never touch the
+// iterate that finalise may select.
 constexpr int BZ = PB / 64;           // planes per step block
 constexpr int BSEG = 64;              // points per plane segment
 constexpr int BZH = BZ + 2 * FDN;     // staged planes
@@ -284,8 +289,12 @@ struct BrickLds {
 };
 
 template <int MH>
-__global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArgs A) {
+__global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArgs A, FinArgs F, int fin) {
   constexpr int ND = MH * (MH + 1) / 2 + MH;
+  if (fin && (int)blockIdx.x == A.nstep) {
+    finalize_richardson(F);
+    return;
+  }
   __shared__ double red[(PB / 64) * (1 + ND)];
   __shared__ BrickLds L;
   const int done = A.S->done;  // stores are skipped after convergence; the loads need not wait for it
@@ -301,11 +310,18 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
     L.x[pl][off] = A.xp[k * nrt + pos];
   }
   if (P.helical) {
+    // T at the staged planes, as theta_kernel forms it
     for (int e = threadIdx.x; e < BZH * BSEG; e += PB) {
       const int pl = e / BSEG, off = e % BSEG;
-      const int k = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
-      const int pos = min(p0 + off, nrt - 1);
-      L.t[pl][off] = A.T[k * nrt + pos];
+      const int kt = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
+      const int pt = min(p0 + off, nrt - 1);
+      const int jt = pt / P.nr, base = kt * nrt + (pt - jt * P.nr);
+      double t = 0;
+#pragma unroll
+      for (int q = 1; q <= FDN; ++q)
+        t = __builtin_fma(P.wt1[q], A.xp[base + wrapi(jt + q, P.nt) * P.nr] - A.xp[base + wrapi(jt - q, P.nt) * P.nr],
+                          t);
+      L.t[pl][off] = t;
     }
   }
   for (int e = threadIdx.x; e < RCOEF * BSEG; e += PB) {
@@ -347,7 +363,7 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
     v[0] = r1 * r1;
     r = r1 / L.c[2 * FDN + 1][lane];
     if (A.hcol >= 0) {
-      xold = A.xo[n];
+      xold = A.xold[n];
       rold = A.fold[n];
     }
   }
@@ -359,7 +375,7 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
       A.DF[(int64_t)A.hcol * N + n] = r - rold;
     }
     if constexpr (MH == 0) {
-      A.xo[n] = __builtin_fma(A.omega, r, x);
+      A.xnew[n] = __builtin_fma(A.omega, r, x);
     } else {
       double df[MH];
 #pragma unroll
@@ -374,7 +390,7 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
     }
   }
   const double s = block_sums<1 + ND, PB>(v, red);
-  if ((int)threadIdx.x < 1 + ND) A.part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  if ((int)threadIdx.x < 1 + ND) A.part[(int64_t)threadIdx.x * A.nstep + blockIdx.x] = s;
 }
 
 // one block: fixed-order sums of an Anderson step's partials, convergence
@@ -467,8 +483,8 @@ static int step_blocks(const PoissonOp& P) {
 
 template <int MH>
 static void launch_anderson(int nblk, int nstep, hipStream_t st, const PoissonOp& P, const StepArgs& A, double beta,
-                            double* xnew) {
-  hipLaunchKernelGGL(aar_step_brick_kernel<MH>, dim3(nstep), dim3(PB), 0, st, P, A);
+                            double* xnew, const FinArgs& F, bool fin) {
+  hipLaunchKernelGGL(aar_step_brick_kernel<MH>, dim3(nstep + (fin ? 1 : 0)), dim3(PB), 0, st, P, A, F, fin ? 1 : 0);
   hipLaunchKernelGGL(aar_finalize_kernel<MH>, dim3(1), dim3(AB), 0, st, (const double*)A.part, nstep, A.count,
                      A.max_iter, A.xbuf, A.tol, A.S);
   hipLaunchKernelGGL(anderson_update_kernel<MH>, dim3(nblk), dim3(PB), 0, st, P, A.xp, (const double*)A.res, xnew,
@@ -540,11 +556,10 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   const int nblk_norm = (int)std::min<int64_t>(ceil_div(N, AB), 1024);
   const int nstep = step_blocks(make_op(ps));
   const int ndmax = 1 + m * (m + 1) / 2 + m;
-  // scratch: xalt | res0 | res1 | T | DX | DF | partials | state | [r1 | u]
+  // scratch: xalt | xalt2 | res0 | res1 | T | DX | DF | partials x 2 | state | [r1 | u]
   const size_t vec = (size_t)N * sizeof(double);
-  const size_t state_end = vec * (4 + 2 * (size_t)m) +
-                           (size_t)std::max({nblk, nblk_norm, nstep}) * ndmax * sizeof(double) + sizeof(AarState) +
-                           256;
+  const int64_t pstride = (int64_t)std::max({nblk, nblk_norm, nstep}) * ndmax;  // doubles per partial buffer
+  const size_t state_end = vec * (5 + 2 * (size_t)m) + 2 * (size_t)pstride * sizeof(double) + sizeof(AarState) + 256;
   const size_t need = state_end + (net ? 2 * vec + (size_t)N * sizeof(float) + 256 : 0);
   if (ps->scratch_bytes < need) {
     if (ps->scratch) (void)hipFree(ps->scratch);
@@ -557,17 +572,21 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
     ps->scratch_bytes = need;
   }
   double* xalt = ps->scratch;
-  double* R[2] = {xalt + N, xalt + 2 * N};
-  double* T = xalt + 3 * N;
-  double* DX = xalt + 4 * N;
+  double* xalt2 = xalt + N;
+  double* R[2] = {xalt + 2 * N, xalt + 3 * N};
+  double* T = xalt + 4 * N;
+  double* DX = xalt + 5 * N;
   double* DF = DX + (int64_t)m * N;
   double* part = DF + (int64_t)m * N;
-  AarState* S =
-      (AarState*)(((uintptr_t)(part + (int64_t)std::max({nblk, nblk_norm, nstep}) * ndmax) + 255) & ~(uintptr_t)255);
+  double* parts[2] = {part, part + pstride};  // partials of odd / even iterations
+  AarState* S = (AarState*)(((uintptr_t)(part + 2 * pstride) + 255) & ~(uintptr_t)255);
   double* r1buf = net ? (double*)(((uintptr_t)ps->scratch + state_end + 255) & ~(uintptr_t)255) : nullptr;
   double* dbuf = net ? r1buf + N : nullptr;
   float* ubuf = net ? (float*)(dbuf + N) : nullptr;
-  double* X[2] = {xalt, x};  // X[1] = caller's x (guess), X[0] = xalt
+  // iterate c lives in X[c % 3] (X[1] = the caller's x, holding the guess):
+  // a step writes X[(c + 1) % 3] while the finalise of step c - 1, which may
+  // select X[(c - 1) % 3], runs in the same launch
+  double* X[3] = {xalt, x, xalt2};
   if (!use_guess) hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, x, N, 1.0);
   CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
   CYX_HIP(hipMemsetAsync(T, 0, vec, st));
@@ -579,9 +598,9 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   PhaseTimer tm(ps->ctx, PH_AAR);
   int count = 1;
   // a Richardson iteration whose finalise has not been enqueued yet: it
-  // rides on the next theta launch, or runs alone before a poll
+  // rides on the next step launch, or runs alone before a poll
   bool fin_pending = false;
-  auto fin_args = [&](int c) { return FinArgs{(const double*)part, nstep, c, max_iter, c & 1, tol, S}; };
+  auto fin_args = [&](int c) { return FinArgs{(const double*)parts[c & 1], nstep, c, max_iter, c % 3, tol, S}; };
   // iterations enqueued between polls of the device flag (the network's
   // kernels do not test it, so the learned path polls more often)
   const int chunk = net ? 6 : 16;
@@ -591,44 +610,44 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       const int xb = count & 1;
       StepArgs A{};
       A.f = f;
-      A.xp = X[xb];
-      A.xo = X[xb ^ 1];
-      A.T = T;
+      A.xp = X[count % 3];
+      A.xold = X[(count + 2) % 3];
+      A.xnew = X[(count + 1) % 3];
       A.res = R[xb];
       A.fold = R[xb ^ 1];
       A.DX = DX;
       A.DF = DF;
-      A.part = part;
+      A.part = parts[xb];
+      A.nstep = nstep;
       A.S = S;
       A.hcol = count > 1 ? (count - 2) % m : -1;
       A.anderson = (count % p == 0) ? 1 : 0;
       A.count = count;
       A.max_iter = max_iter;
-      A.xbuf = xb;
+      A.xbuf = count % 3;
       A.tol = tol;
       A.omega = omega;
-      const ThetaShape ts = P.helical ? theta_shape(P, A.xp, T) : ThetaShape{0, 0};
-      if (ts.ntb > 0 || fin_pending)
-        hipLaunchKernelGGL(theta_kernel, dim3(ts.ntb + (fin_pending ? 1 : 0)), dim3(PB), 0, st, P, A.xp, T, ts.ntb,
-                           ts.pair, fin_args(count - 1));
+      const FinArgs F = fin_args(count - 1);
+      const bool fin = fin_pending;
       fin_pending = false;
       if (!A.anderson) {
-        hipLaunchKernelGGL(aar_step_brick_kernel<0>, dim3(nstep), dim3(PB), 0, st, P, A);
+        hipLaunchKernelGGL(aar_step_brick_kernel<0>, dim3(nstep + (fin ? 1 : 0)), dim3(PB), 0, st, P, A, F,
+                           fin ? 1 : 0);
         fin_pending = true;
       } else {
+        double* xn = X[(count + 1) % 3];
         switch (m) {
-          case 1: launch_anderson<1>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 2: launch_anderson<2>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 3: launch_anderson<3>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 4: launch_anderson<4>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 5: launch_anderson<5>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 6: launch_anderson<6>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          case 7: launch_anderson<7>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
-          default: launch_anderson<8>(nblk, nstep, st, P, A, beta, X[xb ^ 1]); break;
+          case 1: launch_anderson<1>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 2: launch_anderson<2>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 3: launch_anderson<3>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 4: launch_anderson<4>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 5: launch_anderson<5>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 6: launch_anderson<6>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          case 7: launch_anderson<7>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
+          default: launch_anderson<8>(nblk, nstep, st, P, A, beta, xn, F, fin); break;
         }
         if (net) {
           // learned correction of the Anderson iterate, with exact line search
-          double* xn = X[xb ^ 1];
           const AarState* cS = S;
           if (P.helical) {
             const ThetaShape tn = theta_shape(P, xn, T);
@@ -654,7 +673,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       }
     }
     if (fin_pending) {
-      hipLaunchKernelGGL(theta_kernel, dim3(1), dim3(PB), 0, st, P, (const double*)X[(count - 1) & 1], T, 0, 0,
+      hipLaunchKernelGGL(theta_kernel, dim3(1), dim3(PB), 0, st, P, (const double*)X[(count - 1) % 3], T, 0, 0,
                          fin_args(count - 1));
       fin_pending = false;
     }
@@ -668,7 +687,7 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
       return CYX_ERR_NOCONV;
     }
   }
-  if (hs.final_buf == 0) CYX_HIP(hipMemcpyAsync(x, xalt, vec, hipMemcpyDeviceToDevice, st));
+  if (hs.final_buf != 1) CYX_HIP(hipMemcpyAsync(x, X[hs.final_buf], vec, hipMemcpyDeviceToDevice, st));
   CYX_HIP(hipStreamSynchronize(st));
   if (info) {
     info->iterations = hs.iters;

@@ -257,6 +257,28 @@ def test_aar_poisson_parity(device, helical):
     assert np.linalg.norm(f - O.lap_matrix(og) @ phi) / np.linalg.norm(f) <= 1.0001e-8
 
 
+@pytest.mark.parametrize("cap", [3, 4, 5, 6, 7, 8])
+def test_aar_iterate_at_iteration_cap(device, cap):
+    """At the iteration cap AAR returns the iterate whose residual the last
+    iteration measured (src/aar.m:33-72), whichever of the rotating device
+    buffers holds it and whether the last step was Richardson or Anderson
+    (p = 6); the oracle's iterate after the same number of steps."""
+    from cyclix_msparc_amd.api import Poisson
+    from cyclix_msparc_amd.system import GridSpec
+    g = GridSpec(30, 16, 14, 2.0, 0.25, 2 * math.pi / 6 / 16, 0.35, 0.06, 6, 5)
+    og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
+    rng = np.random.default_rng(9)
+    rho = np.abs(rng.standard_normal(g.N))
+    b = -rho.mean() * np.ones(g.N)
+    x_ref, relres_ref, it_ref = O.poisson_solve(og, rho, b, 1e-14, None, cap, "jacobi")
+    dev = device.torch_device
+    phi, info = Poisson(device, g).solve(torch.from_numpy(rho).to(dev), torch.from_numpy(b).to(dev), None, 1e-14, cap)
+    phi = phi.cpu().numpy()
+    assert not info.converged and info.iterations == cap == it_ref
+    assert info.relres == pytest.approx(relres_ref, rel=1e-9)
+    assert np.linalg.norm(phi - x_ref) / np.linalg.norm(x_ref) < 1e-11
+
+
 def test_aar_reports_nonconvergence(device):
     from cyclix_msparc_amd.api import Poisson
     from cyclix_msparc_amd.system import GridSpec
