@@ -66,6 +66,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
+    p.add_argument("--poisson-only", action="store_true",
+                   help="run only the workload's standalone 1e-8 Poisson solve (cold start) and print its own "
+                        "JSON line; for profiling the AAR kernels")
     p.add_argument("--calib-copy", action="store_true",
                    help="after the timed region, run one elementwise copy of 512 MiB (read) + 512 MiB (write) "
                         "as the byte-counter calibration kernel")
@@ -169,9 +172,9 @@ def main():
     N, ns, npl = g.N, sysm.nstates, sysm.npl
     dev = Device(0)
     tdev = dev.torch_device
-    paths = [p for p in args.paths.split(",") if p]
+    paths = [] if args.poisson_only else [p for p in args.paths.split(",") if p]
     unet = None
-    if "mixed" in paths and not args.no_learned:
+    if ("mixed" in paths or args.poisson_only) and not args.no_learned:
         from cyclix_msparc_amd.learned import UNet, default_weights_path
         wp = default_weights_path(g)
         if wp.exists():
@@ -202,6 +205,11 @@ def main():
                       "converged": bool(info8.converged), "relres": float(info8.relres)}
         if mode == "learned":
             unet.detach(poisson)
+    if args.poisson_only:
+        print(json.dumps({"workload": f"{args.workload}: grid {g.nr}x{g.nt}x{g.nz} (N={N}), rho = sum of "
+                                      "atom-centred Gaussians, cold start, AAR m=7 p=6 to 1e-8",
+                          "poisson_1e-8": pois}), flush=True)
+        return
 
     if args.calib_copy:
         src = torch.ones(1 << 26, dtype=torch.float64, device=tdev)
