@@ -602,6 +602,11 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(T* __restrict__ Y, int6
   const int q = blockIdx.x * 256 + threadIdx.x;
   if (q >= nkbn) return;
   const int node = kb_nodes[q];
+  // the node's values are loaded first, so that their latency overlaps the
+  // entry loads and sums
+  T yv[KSC];
+#pragma unroll
+  for (int c = 0; c < KSC; ++c) yv[c] = s0 + c < ncol ? Y[(int64_t)(s0 + c) * ld + node] : zero<T>();
   const uint32_t pk = s_pk[node];
   const int h0 = (int)(pk >> 6), cnt = (int)(pk & 63u);
   T acc[KSC];
@@ -613,9 +618,6 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(T* __restrict__ Y, int6
 #pragma unroll
     for (int c = 0; c < KSC; ++c) acc[c] = cfma(cph, Cs[c * nproj + p], acc[c]);
   }
-  T yv[KSC];
-#pragma unroll
-  for (int c = 0; c < KSC; ++c) yv[c] = s0 + c < ncol ? Y[(int64_t)(s0 + c) * ld + node] : zero<T>();
 #pragma unroll
   for (int c = 0; c < KSC; ++c)
     if (s0 + c < ncol) Y[(int64_t)(s0 + c) * ld + node] = rfma(alpha, acc[c], yv[c]);
@@ -653,14 +655,21 @@ __global__ void __launch_bounds__(256) kb_gather_part_kernel(const T* __restrict
 #pragma unroll
   for (int c = 0; c < GCOL; ++c) acc[c] = zero<T>();
   const T* y = Y + (int64_t)s0 * ld;
-  for (int e = c0 + threadIdx.x; e < c1; e += 256) {
-    const int node = g_node[e];
-    const T gc = g_coef[e];
+  // the next entry's node index and coefficient are loaded one step ahead
+  int e = c0 + threadIdx.x;
+  int node = e < c1 ? g_node[e] : 0;
+  T gc = e < c1 ? g_coef[e] : zero<T>();
+  for (; e < c1; e += 256) {
+    const int en = e + 256;
+    const int node_n = en < c1 ? g_node[en] : 0;
+    const T gc_n = en < c1 ? g_coef[en] : zero<T>();
     T yv[GCOL];
 #pragma unroll
     for (int c = 0; c < GCOL; ++c) yv[c] = c < nc ? y[(int64_t)c * ld + node] : zero<T>();
 #pragma unroll
     for (int c = 0; c < GCOL; ++c) acc[c] = cfma(gc, yv[c], acc[c]);
+    node = node_n;
+    gc = gc_n;
   }
   double v[K];
 #pragma unroll
