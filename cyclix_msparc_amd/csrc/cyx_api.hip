@@ -216,7 +216,7 @@ static void build_kb(const cyx_projector_desc* d, const double* W, const double 
   }
   // gather chunks: node ranges holding about KB_CHUNK_ENTRIES entries of all
   // projectors together (cnt[n + 1] is node n's entry count)
-  constexpr int KB_CHUNK_ENTRIES = 4096;
+  constexpr int KB_CHUNK_ENTRIES = 2048;
   std::vector<int64_t> bound{0};
   for (int64_t n = 0, acc = 0; n < N; ++n) {
     acc += cnt[n + 1];
