@@ -23,6 +23,8 @@
 // iteration after convergence store nothing, so the host enqueues iterations
 // in chunks and polls a device flag between chunks.  Every reduction has a
 // fixed order: a solve is bitwise reproducible.
+#include <cmath>
+
 #include "cyx_internal.hpp"
 #include "cyx_reduce.hpp"
 
@@ -602,8 +604,15 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   bool fin_pending = false;
   auto fin_args = [&](int c) { return FinArgs{(const double*)parts[c & 1], nstep, c, max_iter, c % 3, tol, S}; };
   // iterations enqueued between polls of the device flag (the network's
-  // kernels do not test it, so the learned path polls more often)
-  const int chunk = net ? 6 : 16;
+  // kernels do not test it, so the learned path polls every 6).  A poll costs
+  // an idle gap of tens of microseconds, an iteration enqueued past
+  // convergence a no-op launch of ~16 us, so the chunk follows the predicted
+  // remaining count: log-residual slope between the last two polls, 8..64
+  // iterations, 32 until a slope is known.
+  int chunk = net ? 6 : 32;
+  double prev_lr = 0.0;
+  int prev_it = 0;
+  bool have_prev = false;
   while (count <= max_iter) {
     const int cend = std::min(max_iter, count + chunk - 1);
     for (; count <= cend; ++count) {
@@ -685,6 +694,17 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
     if (net && !(hs.relres <= 1e3)) {
       *diverged = true;
       return CYX_ERR_NOCONV;
+    }
+    if (!net && hs.relres > 0.0 && hs.iters > 0) {
+      const double lr = std::log(hs.relres);
+      chunk = 32;
+      if (have_prev && hs.iters > prev_it) {
+        const double slope = (lr - prev_lr) / (hs.iters - prev_it);
+        if (slope < 0.0) chunk = (int)std::min(64.0, std::max(8.0, std::ceil((std::log(tol) - lr) / slope) + 2.0));
+      }
+      prev_lr = lr;
+      prev_it = hs.iters;
+      have_prev = true;
     }
   }
   if (hs.final_buf != 1) CYX_HIP(hipMemcpyAsync(x, X[hs.final_buf], vec, hipMemcpyDeviceToDevice, st));
