@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per recurrence launch of the fused stencil at this workload, from
 # the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
 # calibration) recorded in profiles/scf_step_cnt10_helical.md
-MEASURED_TRAFFIC = {("cnt10_helical", 16): 3.6670e+09, ("cnt10_helical", 8): 1.8463e+09}
+MEASURED_TRAFFIC = {("cnt10_helical", 16): 3.6670e+09, ("cnt10_helical", 8): 1.8445e+09}
 METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM peak"
 
 
