@@ -273,8 +273,11 @@ struct StepArgs {
 // 8 zc .. 8 zc + 7, wave w one plane.  LDS holds x at planes k - FDN ..
 // k + FDN of the block's planes over the segment and an r halo of FDN on
 // each side, the helical theta derivative T = (-2 tau D_theta) x at the same
-// planes over the segment (computed here from x), and the 14 coefficients
-// of the segment's radii.  Halo positions outside the plane are clamped into
+// planes over the segment and the theta second-derivative sum of the block's
+// own planes (both formed here from the same theta-neighbour loads), and the
+// 14 coefficients of the segment's radii.  The per-point vectors (f, the
+// previous iterate and residual, the Anderson history) are loaded before the
+// staging barrier, so their latency overlaps it.  Halo positions outside the plane are clamped into
 // it: they only meet radial taps whose coefficient is zero.  With fin set,
 // one block more (the last) finalises the previous Richardson iteration; the
 // iterates are triple-buffered, so this step's stores never touch the
@@ -287,6 +290,7 @@ constexpr int BXW = BSEG + 2 * FDN;   // staged x row with r halo
 struct BrickLds {
   double x[BZH][BXW];
   double t[BZH][BSEG];
+  double s[BZ][BSEG];  // sum_p w2_p (x(j+p) + x(j-p)) of the block's planes
   double c[RCOEF][BSEG];
 };
 
@@ -305,25 +309,57 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
   const int nseg = (nrt + BSEG - 1) / BSEG;
   const int seg = blockIdx.x % nseg, k0 = (blockIdx.x / nseg) * BZ;
   const int p0 = seg * BSEG;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = k0 + w, pos = p0 + lane;
+  const bool ok = k < P.nz && pos < nrt;
+  const int n = k * nrt + pos;
+  // per-point loads first (they do not depend on the staging)
+  double fv = 0, xold = 0, rold = 0;
+  double dfo[MH > 0 ? MH : 1];
+  if (ok) {
+    fv = A.f[n];
+    if (A.hcol >= 0) {
+      xold = A.xold[n];
+      rold = A.fold[n];
+    }
+    if constexpr (MH > 0) {
+#pragma unroll
+      for (int a = 0; a < MH; ++a) dfo[a] = A.DF[(int64_t)a * N + n];
+    }
+  }
   for (int e = threadIdx.x; e < BZH * BXW; e += PB) {
     const int pl = e / BXW, off = e % BXW;
     const int k = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
     const int pos = min(max(p0 - FDN + off, 0), nrt - 1);
     L.x[pl][off] = A.xp[k * nrt + pos];
   }
-  if (P.helical) {
-    // T at the staged planes, as theta_kernel forms it
-    for (int e = threadIdx.x; e < BZH * BSEG; e += PB) {
-      const int pl = e / BSEG, off = e % BSEG;
-      const int kt = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
-      const int pt = min(p0 + off, nrt - 1);
-      const int jt = pt / P.nr, base = kt * nrt + (pt - jt * P.nr);
+  // T at the staged planes (helical), as theta_kernel forms it, and the
+  // theta sum at the block's own planes, from one set of neighbour loads;
+  // planes past nz (a partial last brick) are wrapped, never emitted
+  for (int e = threadIdx.x; e < BZH * BSEG; e += PB) {
+    const int pl = e / BSEG, off = e % BSEG;
+    const bool own = pl >= FDN && pl < FDN + BZ;
+    if (!P.helical && !own) continue;
+    const int kt = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
+    const int pt = min(p0 + off, nrt - 1);
+    const int jt = pt / P.nr, base = kt * nrt + (pt - jt * P.nr);
+    double xu[FDN], xd[FDN];
+#pragma unroll
+    for (int q = 1; q <= FDN; ++q) {
+      xu[q - 1] = A.xp[base + wrapi(jt + q, P.nt) * P.nr];
+      xd[q - 1] = A.xp[base + wrapi(jt - q, P.nt) * P.nr];
+    }
+    if (P.helical) {
       double t = 0;
 #pragma unroll
-      for (int q = 1; q <= FDN; ++q)
-        t = __builtin_fma(P.wt1[q], A.xp[base + wrapi(jt + q, P.nt) * P.nr] - A.xp[base + wrapi(jt - q, P.nt) * P.nr],
-                          t);
+      for (int q = 1; q <= FDN; ++q) t = __builtin_fma(P.wt1[q], xu[q - 1] - xd[q - 1], t);
       L.t[pl][off] = t;
+    }
+    if (own) {
+      double st = 0;
+#pragma unroll
+      for (int q = 1; q <= FDN; ++q) st = __builtin_fma(P.w2[q], xu[q - 1] + xd[q - 1], st);
+      L.s[pl - FDN][off] = st;
     }
   }
   for (int e = threadIdx.x; e < RCOEF * BSEG; e += PB) {
@@ -331,16 +367,11 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
     const int pos = min(p0 + off, nrt - 1);
     L.c[cc][off] = P.coefT[cc * P.nr + pos % P.nr];
   }
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int k = k0 + w, pos = p0 + lane;
-  const bool ok = k < P.nz && pos < nrt;
-  const int j = pos / P.nr, i = pos - j * P.nr;
-  const int n = k * nrt + pos;
   __syncthreads();
   double v[1 + ND];
 #pragma unroll
   for (int d = 0; d < 1 + ND; ++d) v[d] = 0;
-  double x = 0, r = 0, xold = 0, rold = 0;
+  double x = 0, r = 0;
   if (ok) {
     const int xc = FDN + lane, zc = FDN + w;
     x = L.x[zc][xc];
@@ -350,24 +381,15 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
       acc = __builtin_fma(L.c[p - 1][lane], L.x[zc][xc + p], acc);
       acc = __builtin_fma(L.c[FDN + p - 1][lane], L.x[zc][xc - p], acc);
     }
-    const int rowk = k * nrt + i;
-    double st = 0.0;
-#pragma unroll
-    for (int p = 1; p <= FDN; ++p)
-      st = __builtin_fma(P.w2[p], A.xp[rowk + wrapi(j + p, P.nt) * P.nr] + A.xp[rowk + wrapi(j - p, P.nt) * P.nr], st);
-    acc = __builtin_fma(L.c[2 * FDN][lane], st, acc);
+    acc = __builtin_fma(L.c[2 * FDN][lane], L.s[w][lane], acc);
 #pragma unroll
     for (int q = 1; q <= FDN; ++q) {
       acc = __builtin_fma(P.wz2[q], L.x[zc + q][xc] + L.x[zc - q][xc], acc);
       if (P.helical) acc = __builtin_fma(P.wz1[q], L.t[zc + q][lane] - L.t[zc - q][lane], acc);
     }
-    const double r1 = A.f[n] - acc;
+    const double r1 = fv - acc;
     v[0] = r1 * r1;
     r = r1 / L.c[2 * FDN + 1][lane];
-    if (A.hcol >= 0) {
-      xold = A.xold[n];
-      rold = A.fold[n];
-    }
   }
   if (done) return;
   if (ok) {
@@ -381,7 +403,7 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
     } else {
       double df[MH];
 #pragma unroll
-      for (int a = 0; a < MH; ++a) df[a] = a == A.hcol ? r - rold : A.DF[(int64_t)a * N + n];
+      for (int a = 0; a < MH; ++a) df[a] = a == A.hcol ? r - rold : dfo[a];
       int d = 1;
 #pragma unroll
       for (int a = 0; a < MH; ++a)
