@@ -294,8 +294,17 @@ struct BrickLds {
   double c[RCOEF][BSEG];
 };
 
+// Richardson and short-history Anderson steps are held to 80 VGPRs, so that
+// three blocks share a CU and a production grid (~540 blocks) is resident at
+// once; longer histories need more registers for their block sums
 template <int MH>
-__global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArgs A, FinArgs F, int fin) {
+constexpr int brick_waves_per_eu() {
+  return MH < 5 ? 6 : 1;
+}
+
+template <int MH>
+__global__ void __launch_bounds__(PB) __attribute__((amdgpu_waves_per_eu(brick_waves_per_eu<MH>())))
+aar_step_brick_kernel(PoissonOp P, StepArgs A, FinArgs F, int fin) {
   constexpr int ND = MH * (MH + 1) / 2 + MH;
   if (fin && (int)blockIdx.x == A.nstep) {
     finalize_richardson(F);
@@ -327,45 +336,80 @@ __global__ void __launch_bounds__(PB) aar_step_brick_kernel(PoissonOp P, StepArg
       for (int a = 0; a < MH; ++a) dfo[a] = A.DF[(int64_t)a * N + n];
     }
   }
-  for (int e = threadIdx.x; e < BZH * BXW; e += PB) {
+  // staging: every load is issued unconditionally (indices clamped into
+  // range, only the LDS stores are predicated) and the x and coefficient
+  // values are stored after the theta work, so that they travel with the
+  // first trip's theta loads: a helical block waits on three memory round
+  // trips (one per theta trip) instead of one per trip of every loop
+  constexpr int NXL = (BZH * BXW + PB - 1) / PB;
+  constexpr int NCL = (RCOEF * BSEG + PB - 1) / PB;
+  constexpr int NTL = (BZH * BSEG + PB - 1) / PB;
+  double xr[NXL], cr[NCL];
+#pragma unroll
+  for (int it = 0; it < NXL; ++it) {
+    const int e = min((int)threadIdx.x + it * PB, BZH * BXW - 1);
     const int pl = e / BXW, off = e % BXW;
-    const int k = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
-    const int pos = min(max(p0 - FDN + off, 0), nrt - 1);
-    L.x[pl][off] = A.xp[k * nrt + pos];
+    const int kx = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
+    const int px = min(max(p0 - FDN + off, 0), nrt - 1);
+    xr[it] = A.xp[kx * nrt + px];
   }
-  // T at the staged planes (helical), as theta_kernel forms it, and the
-  // theta sum at the block's own planes, from one set of neighbour loads;
-  // planes past nz (a partial last brick) are wrapped, never emitted
-  for (int e = threadIdx.x; e < BZH * BSEG; e += PB) {
-    const int pl = e / BSEG, off = e % BSEG;
-    const bool own = pl >= FDN && pl < FDN + BZ;
-    if (!P.helical && !own) continue;
+#pragma unroll
+  for (int it = 0; it < NCL; ++it) {
+    const int e = min((int)threadIdx.x + it * PB, RCOEF * BSEG - 1);
+    const int cc = e / BSEG, off = e % BSEG;
+    cr[it] = P.coefT[cc * P.nr + min(p0 + off, nrt - 1) % P.nr];
+  }
+  // theta neighbours of staged point (pl, off) (planes past nz, in a partial
+  // last brick, are wrapped and never emitted)
+  auto theta_loads = [&](int pl, int off, double (&xu)[FDN], double (&xd)[FDN]) {
     const int kt = ((k0 - FDN + pl) % P.nz + P.nz) % P.nz;
     const int pt = min(p0 + off, nrt - 1);
     const int jt = pt / P.nr, base = kt * nrt + (pt - jt * P.nr);
-    double xu[FDN], xd[FDN];
 #pragma unroll
     for (int q = 1; q <= FDN; ++q) {
       xu[q - 1] = A.xp[base + wrapi(jt + q, P.nt) * P.nr];
       xd[q - 1] = A.xp[base + wrapi(jt - q, P.nt) * P.nr];
     }
-    if (P.helical) {
-      double t = 0;
+  };
+  auto theta_sum = [&](const double (&xu)[FDN], const double (&xd)[FDN]) {
+    double st = 0;
 #pragma unroll
-      for (int q = 1; q <= FDN; ++q) t = __builtin_fma(P.wt1[q], xu[q - 1] - xd[q - 1], t);
-      L.t[pl][off] = t;
-    }
-    if (own) {
-      double st = 0;
+    for (int q = 1; q <= FDN; ++q) st = __builtin_fma(P.w2[q], xu[q - 1] + xd[q - 1], st);
+    return st;
+  };
+  if (P.helical) {
+    // T at every staged plane, as theta_kernel forms it, and the theta sum
+    // at the block's own planes, from one set of neighbour loads
 #pragma unroll
-      for (int q = 1; q <= FDN; ++q) st = __builtin_fma(P.w2[q], xu[q - 1] + xd[q - 1], st);
-      L.s[pl - FDN][off] = st;
+    for (int it = 0; it < NTL; ++it) {
+      const int e0 = (int)threadIdx.x + it * PB;
+      const int e = min(e0, BZH * BSEG - 1);
+      const int pl = e / BSEG, off = e % BSEG;
+      double xu[FDN], xd[FDN];
+      theta_loads(pl, off, xu, xd);
+      if (e0 < BZH * BSEG) {
+        double t = 0;
+#pragma unroll
+        for (int q = 1; q <= FDN; ++q) t = __builtin_fma(P.wt1[q], xu[q - 1] - xd[q - 1], t);
+        L.t[pl][off] = t;
+        if (pl >= FDN && pl < FDN + BZ) L.s[pl - FDN][off] = theta_sum(xu, xd);
+      }
     }
+  } else {
+    // the theta sum at the block's own planes: one point per thread
+    double xu[FDN], xd[FDN];
+    theta_loads(FDN + w, lane, xu, xd);
+    L.s[w][lane] = theta_sum(xu, xd);
   }
-  for (int e = threadIdx.x; e < RCOEF * BSEG; e += PB) {
-    const int cc = e / BSEG, off = e % BSEG;
-    const int pos = min(p0 + off, nrt - 1);
-    L.c[cc][off] = P.coefT[cc * P.nr + pos % P.nr];
+#pragma unroll
+  for (int it = 0; it < NXL; ++it) {
+    const int e = threadIdx.x + it * PB;
+    if (e < BZH * BXW) L.x[e / BXW][e % BXW] = xr[it];
+  }
+#pragma unroll
+  for (int it = 0; it < NCL; ++it) {
+    const int e = threadIdx.x + it * PB;
+    if (e < RCOEF * BSEG) L.c[e / BSEG][e % BSEG] = cr[it];
   }
   __syncthreads();
   double v[1 + ND];
