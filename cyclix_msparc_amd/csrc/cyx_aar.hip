@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(AB) aar_finalize_kernel(const double* part, in
   }
   if (lane < MH) W.b[lane] = tot[1 + MH * (MH + 1) / 2 + lane];
   wave_sync();
-  if (!chol_solve_wave<MH>(W, lane)) {
+  if (!chol_solve_regs<MH>(W, lane)) {
     pinv_solve_wave<MH>(W, lane);
     if (lane == 0) S->pinv_eig += 1;
   }

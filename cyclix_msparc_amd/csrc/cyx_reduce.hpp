@@ -223,7 +223,7 @@ __device__ inline void column_sums(const double* __restrict__ part, int nblk, in
 template <int M>
 struct PinvShared {
   static constexpr int ME = M + (M & 1);
-  double A[M * M], V[M * M], Li[M * M], b[M], vb[M], out[M];
+  double A[M * M], V[M * M], b[M], vb[M], out[M];
 };
 
 template <int M>
@@ -311,60 +311,80 @@ __device__ void pinv_solve_wave(PinvShared<M>& W, const int lane) {
 
 // pinv(A) b for a symmetric positive semi-definite A (M x M in W.A, b in
 // W.b) when A is certified full rank under pinv's tolerance: Cholesky A = L
-// L^T on one wavefront, then lambda_min(A) = 1 / ||L^{-1}||_2^2 >= 1 /
-// ||L^{-1}||_F^2 > 100 M trace(A) eps >= 100 M lambda_max eps, so pinv keeps
-// every eigenvalue and pinv(A) b = A^{-1} b = L^{-T} L^{-1} b.  Returns false
-// (W.out untouched, W.A intact) when the certificate fails; the caller then
-// takes pinv_solve_wave.  Every branch is uniform across the wave.
+// L^T, then lambda_min(A) = 1 / ||L^{-1}||_2^2 >= 1 / ||L^{-1}||_F^2 > 100 M
+// trace(A) eps >= 100 M lambda_max eps, so pinv keeps every eigenvalue and
+// pinv(A) b = A^{-1} b = L^{-T} L^{-1} b.  Returns false (W.out untouched,
+// W.A intact) when the certificate fails; the caller then takes
+// pinv_solve_wave.  Every lane of the wavefront computes the factor, its
+// inverse and the two triangular solves redundantly in registers (identical
+// operations, so identical results and uniform branches); the only
+// cross-lane step is the Frobenius-norm wave_sum, where lane e holds element
+// e of L^{-1} in row-major order.  (An LDS form with three wavefront syncs
+// per elimination step took 11.8 us per Anderson finalise at m = 7, this one
+// 9.2 us.)
 template <int M>
-__device__ bool chol_solve_wave(PinvShared<M>& W, const int lane) {
-  static_assert(M <= 64, "one lane per row");
-  for (int e = lane; e < M * M; e += 64) W.V[e] = W.A[e];
-  wave_sync();
+__device__ bool chol_solve_regs(PinvShared<M>& W, const int lane) {
+  static_assert(M * M <= 64, "one lane per element of the inverse factor");
+  double V[M][M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) V[i][j] = j <= i ? W.A[i * M + j] : 0.0;
   double tr = 0;
+#pragma unroll
   for (int i = 0; i < M; ++i) tr += W.A[i * M + i];
+#pragma unroll
   for (int k = 0; k < M; ++k) {  // right-looking Cholesky, L in the lower triangle of V
-    const double d = W.V[k * M + k];
+    const double d = V[k][k];
     if (!(d > 0.0)) return false;
     const double lk = sqrt(d), inv = 1.0 / lk;
-    double lik = 0;
-    if (lane > k && lane < M) lik = W.V[lane * M + k] * inv;
-    wave_sync();
-    if (lane > k && lane < M) W.V[lane * M + k] = lik;
-    if (lane == k) W.V[k * M + k] = lk;
-    wave_sync();
-    for (int e = lane; e < M * M; e += 64) {
-      const int i = e / M, j = e % M;
-      if (j > k && i >= j) W.V[e] -= W.V[i * M + k] * W.V[j * M + k];
-    }
-    wave_sync();
+#pragma unroll
+    for (int i = k + 1; i < M; ++i) V[i][k] = V[i][k] * inv;
+    V[k][k] = lk;
+#pragma unroll
+    for (int j = k + 1; j < M; ++j)
+#pragma unroll
+      for (int i = j; i < M; ++i) V[i][j] -= V[i][k] * V[j][k];
   }
-  if (lane < M) {  // column c = lane of L^{-1}, by forward substitution (own-lane data only)
-    const int c = lane;
+  double Li[M][M];  // L^{-1} by forward substitution, column by column
+#pragma unroll
+  for (int c = 0; c < M; ++c)
+#pragma unroll
     for (int i = 0; i < M; ++i) {
       double s = i == c ? 1.0 : 0.0;
       if (i >= c) {
-        for (int kk = c; kk < i; ++kk) s -= W.V[i * M + kk] * W.Li[kk * M + c];
-        s /= W.V[i * M + i];
+#pragma unroll
+        for (int kk = c; kk < i; ++kk) s -= V[i][kk] * Li[kk][c];
+        s /= V[i][i];
       }
-      W.Li[i * M + c] = s;
+      Li[i][c] = s;
     }
-  }
-  wave_sync();
+  double mine = 0;  // lane e holds element e of L^{-1} (row-major)
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int c = 0; c < M; ++c)
+      if (lane == i * M + c) mine = Li[i][c];
   double f2 = 0;
-  for (int e = lane; e < M * M; e += 64) f2 += W.Li[e] * W.Li[e];
+  f2 += mine * mine;
   f2 = wave_sum(f2);
   if (!(f2 > 0.0) || !(1.0 / f2 > 100.0 * M * tr * 2.220446049250313e-16)) return false;
-  if (lane < M) {  // y = L^{-1} b
-    double y = 0;
-    for (int c = 0; c <= lane; ++c) y += W.Li[lane * M + c] * W.b[c];
-    W.vb[lane] = y;
+  double bb[M], y[M];
+#pragma unroll
+  for (int c = 0; c < M; ++c) bb[c] = W.b[c];
+#pragma unroll
+  for (int l = 0; l < M; ++l) {  // y = L^{-1} b
+    double t = 0;
+#pragma unroll
+    for (int c = 0; c <= l; ++c) t += Li[l][c] * bb[c];
+    y[l] = t;
   }
-  wave_sync();
-  if (lane < M) {  // x = L^{-T} y
-    double x = 0;
-    for (int i = lane; i < M; ++i) x += W.Li[i * M + lane] * W.vb[i];
-    W.out[lane] = x;
+#pragma unroll
+  for (int l = 0; l < M; ++l) {  // x = L^{-T} y
+    double t = 0;
+#pragma unroll
+    for (int i = l; i < M; ++i) t += Li[i][l] * y[i];
+    if (lane == l) W.out[l] = t;
   }
   wave_sync();
   return true;
