@@ -7,10 +7,13 @@ Workload: (10,0) carbon nanotube, 1/5 sector with helical twist 0.05 rad/Bohr,
 eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24.
 
 Two paths are timed: the FP64 path (complex128 throughout, Jacobi-AAR) and
-the mixed path (complex64 psi, filter and H-applies with FP64 accumulation of
-the Gram products, FP64 dense eigensolve, density and energies; with
---learned also the learned Poisson correction).  `value` is the mixed path's
-time per step; `speedup_vs_fp64` compares the two.  The standalone Poisson
+the mixed path (complex64 psi and Chebyshev filter; the Rayleigh-Ritz H psi
+in FP64 arithmetic, FP64 accumulation of the Gram products, FP64 dense
+eigensolve, density and energies; with --learned also the learned Poisson
+correction).  `value` is the mixed path's time per step; `speedup_vs_fp64`
+compares the two.  `error_vs_fp64` compares the two paths' converged
+energies, and their occupied eigenvalues after both final blocks take more
+CheFSI passes on the FP64 path's converged potential.  The standalone Poisson
 solve is timed with Jacobi-AAR and, when trained weights for the grid exist,
 with the learned correction.
 
@@ -19,7 +22,12 @@ a 1e-5 Ha/atom energy target (src/initialization.m:516-552): CheFSI pass
 (filter + Rayleigh-Ritz + rotation + W-normalisation, src/eigSolver.m:98-124),
 occupations, electron density, AAR Poisson solve to poisson_tol (warm
 start, src/poissonSolve.m), LDA-PW Vxc and Veff, the energy grid sums and
-Periodic-Pulay potential mixing.  Inputs are resident in HBM when the timed
+Periodic-Pulay potential mixing; the first iteration of a solve also holds
+the Lanczos bounds and the rhoTrigger - 1 Chebyshev cycles before the first
+density (src/scf.m:111-130).  The K timed steps are iterations of complete
+SCF solves from the same cold start, restarted whenever a solve converges,
+so none is a post-convergence iteration; `time_to_converge` is one more
+solve from the cold start.  Inputs are resident in HBM when the timed
 region starts.  The workload's standalone Poisson solve (atom-Gaussian
 density, cold start, 1e-8 relative residual) is timed separately.
 
@@ -52,8 +60,8 @@ METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM p
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="cnt10_helical")
     p.add_argument("--paths", default="fp64,mixed",
                    help="comma list of fp64 (complex128, Jacobi-AAR) and mixed (complex64 filter and subspace "
@@ -76,17 +84,19 @@ def parse():
 
 
 def run_path(dev, sysm, args, precision, unet=None):
-    """W warm-up and K timed SCF iterations of one path; returns its record."""
+    """One path: W warm-up SCF iterations (untimed), then K timed SCF
+    iterations of SCF solves from the cold start, each solve restarted from
+    the same state when it converges, so no timed iteration is a
+    post-convergence one; then one more full solve from the cold start for
+    the time to converge.  Returns (record, state)."""
     import torch
 
     from cyclix_msparc_amd.scf import GroundState, SCFOptions
     g = sysm.grid
     N, ns, npl = g.N, sysm.nstates, sysm.npl
     bsz = 16 if precision == "fp64" else 8
-    # SCF_tol for 1e-5 Ha/atom (src/initialization.m:516-545), poisson_tol = 0.01 SCF_tol;
-    # MINIT_SCF keeps the loop running through warm-up + timed iterations
-    opts = SCFOptions(target_energy_accuracy=1e-5, MINIT_SCF=args.warmup + args.steps,
-                      MAXIT_SCF=max(100, args.warmup + args.steps), MAXIT_POISSON=args.aar_maxit)
+    # SCF_tol for 1e-5 Ha/atom (src/initialization.m:516-545), poisson_tol = 0.01 SCF_tol
+    opts = SCFOptions(target_energy_accuracy=1e-5, MAXIT_POISSON=args.aar_maxit)
     gs = GroundState(dev, sysm, options=opts, precision=precision)
     if unet is not None:
         unet.attach(gs.poisson)
@@ -96,26 +106,46 @@ def run_path(dev, sysm, args, precision, unet=None):
         X = X.astype(np.complex64)
     gs.init_psi([X])
     del X
+    # warm-up: the first SCF iterations of a solve from the cold start (the
+    # first includes the Lanczos bounds and the rhoTrigger Chebyshev cycles,
+    # src/scf.m:111-130)
     it = gs.iterations()
-    # warm-up: the first pass with its Lanczos bounds, the Chebyshev cycles
-    # before rhoTrigger and `warmup` SCF iterations (src/scf.m:111-125)
     for _ in range(args.warmup):
-        next(it)
+        if next(it, None) is None:
+            break
+    it.close()
     dev.synchronize()
     torch.cuda.synchronize()
     dev.set_timing(True)
     dev.reset_timing()
-    recs = []
+    gs.restart()
+    recs, restarts = [], 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        recs.append(next(it))
+    it = gs.iterations()
+    while len(recs) < args.steps:
+        rec = next(it, None)
+        if rec is None:          # converged: start the next solve from the cold start
+            gs.restart()
+            restarts += 1
+            it = gs.iterations()
+            continue
+        recs.append(rec)
     dev.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    it.close()
     ms_step = (t1 - t0) * 1e3 / args.steps
     tm = dev.timing()
     dev.set_timing(False)
-    it.close()
+    # time to converge: one full solve from the cold start
+    gs.restart()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = gs.run()
+    torch.cuda.synchronize()
+    ttc_ms = (time.perf_counter() - t) * 1e3
+    resid = gs.residuals()[0]
+    nocc = int(round(sysm.nelectron / 2))
     # roofline of the dominant kernel, the fused local stencil + three-term step:
     # algorithmic bytes per launch = read Y + read X_prev + write Y_new (ns
     # columns of N points, bsz bytes) + Veff (N doubles, shared by columns)
@@ -123,17 +153,21 @@ def run_path(dev, sysm, args, precision, unet=None):
     bytes_launch = 3.0 * N * ns * bsz + 8.0 * N
     avg_ms = st_ms / max(st_n, 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    # whole filter: B_filter = (3m-1) N ns b + m N 8; its time is the m-1
-    # stencil launches, the first-step H-apply and the m KB gathers of the
-    # filter (Rayleigh-Ritz adds one H-apply and one gather per pass)
+    # whole filter: B_filter = (3m-1) N ns b + m N 8 over the m-1 stencil
+    # launches, the first-step H-apply and the m KB gathers/scatters of each
+    # filter (the Rayleigh-Ritz H-apply, "rr_apply", and its gather are not
+    # filter time; the Lanczos H-applies of each solve's first pass are in
+    # h_apply, a few single-column launches)
     ha_ms, _ = tm["h_apply"]
     kb_ms, _ = tm["kb_gather"]
-    filt_ms = (st_ms + ha_ms / 2 + kb_ms * npl / (npl + 1)) / args.steps
+    n_pass = max(st_n // max(npl - 1, 1), 1)
+    filt_ms_pass = (st_ms + ha_ms + kb_ms * npl / (npl + 1)) / n_pass
     b_filter = (3 * npl - 1) * N * ns * bsz + npl * N * 8.0
-    filter_gbs = b_filter / (filt_ms * 1e-3) / 1e9
+    filter_gbs = b_filter / (filt_ms_pass * 1e-3) / 1e9
     rec = {
         "ms_per_step": round(ms_step, 3),
-        "dtype": "complex128" if precision == "fp64" else "complex64 (FP64 accumulation, FP64 dense eig/energies)",
+        "dtype": "complex128" if precision == "fp64" else
+                 "complex64 filter (FP64 Rayleigh-Ritz H-apply, FP64 Gram accumulation, FP64 dense eig/energies)",
         "learned_poisson": unet is not None,
         "roofline": {"bound": "hbm", "kernel": f"stencil_step_kernel<{'c128' if precision == 'fp64' else 'c64'}> "
                                                "(fused local H-apply + Chebyshev step; the KB term is its own gather/scatter launches)",
@@ -141,19 +175,47 @@ def run_path(dev, sysm, args, precision, unet=None):
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((args.workload, bsz)),
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4)},
-        "filter": {"ms": round(filt_ms, 3), "GB/s": round(filter_gbs, 1),
-                   "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4)},
+        "filter": {"ms_per_pass": round(filt_ms_pass, 3), "GB/s": round(filter_gbs, 1),
+                   "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4), "passes": n_pass},
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in tm.items()},
         "scf": {"err": [round(r.err, 6) for r in recs], "Eatom": [round(r.Etotal / sysm.n_atm, 8) for r in recs],
-                "aar_iterations": [r.aar_iterations for r in recs]},
+                "aar_iterations": [r.aar_iterations for r in recs], "restarts": restarts,
+                "post_convergence_steps": 0},
+        "time_to_converge": {"ms": round(ttc_ms, 2), "scf_iterations": res.count_SCF,
+                             "chefsi_passes": gs.chefsi_passes, "converged": bool(res.converged),
+                             "Eatom": res.Eatom, "SCF_tol": gs.opt.SCF_tol,
+                             "max_occupied_residual_fp64": float(np.max(resid[:nocc]))},
     }
-    state = dict(bounds=gs.bounds, veff=gs.veff.cpu().numpy(),
-                 aar_its=float(np.mean([r.aar_iterations for r in recs])))
+    state = dict(bounds=gs.bounds, veff=gs.veff.cpu().numpy(), aar_its=float(np.mean([r.aar_iterations for r in recs])),
+                 gs=gs, Eatom=res.Eatom, nocc=nocc)
     if unet is not None:
         unet.detach(gs.poisson)
-    del gs
-    torch.cuda.empty_cache()
     return rec, state
+
+
+def precision_check(states, npl, passes=3):
+    """Mixed against FP64 on one Hamiltonian: both final blocks take the FP64
+    path's converged Veff and `passes` more CheFSI passes; returns the
+    occupied-eigenvalue and energy differences and the FP64 residuals."""
+    ref = states["fp64"]
+    veff = ref["gs"].veff.clone()
+    out = {"dEatom_converged": abs(states["mixed"]["Eatom"] - ref["Eatom"])}
+    ev, resid = {}, {}
+    for path in ("fp64", "mixed"):
+        gs = states[path]["gs"]
+        h = gs.hams[0]
+        h.set_veff(veff)
+        a0, bup, cut = (float(v[0]) for v in gs.bounds)
+        for _ in range(passes):
+            e = h.eig_pass(gs.psi[0], npl, cut, bup, a0, work=gs.work[0])
+            a0, cut = float(np.min(e)), float(np.max(e)) + 0.1
+        ev[path] = e
+        resid[path] = h.ritz_residuals(gs.psi[0], e, gs.work[0])
+    nocc = ref["nocc"]
+    out["max_occupied_dlambda"] = float(np.max(np.abs(ev["mixed"][:nocc] - ev["fp64"][:nocc])))
+    out["max_occupied_residual_fp64"] = {p: float(np.max(resid[p][:nocc])) for p in resid}
+    out["passes_on_common_veff"] = passes
+    return out
 
 
 def main():
@@ -209,6 +271,10 @@ def main():
         print(json.dumps({"workload": f"{args.workload}: grid {g.nr}x{g.nt}x{g.nz} (N={N}), rho = sum of "
                                       "atom-centred Gaussians, cold start, AAR m=7 p=6 to 1e-8",
                           "poisson_1e-8": pois}), flush=True)
+        poisson.close()
+        if unet is not None:
+            unet.close()
+        dev.close()
         return
 
     if args.calib_copy:
@@ -219,6 +285,10 @@ def main():
 
     head = "mixed" if "mixed" in results else paths[0]
     H = results[head]
+    err_fp64 = precision_check(states, npl) if ("fp64" in states and "mixed" in states) else None
+    for st in states.values():
+        st["gs"].close()
+    poisson.close()
     cpu = None
     if not args.no_cpu_baseline:
         st = states["fp64"] if "fp64" in states else states[head]
@@ -253,10 +323,18 @@ def main():
         "paths": results,
         "speedup_vs_fp64": (round(results["fp64"]["ms_per_step"] / H["ms_per_step"], 3)
                             if "fp64" in results and head != "fp64" else None),
+        "time_to_converge": H["time_to_converge"],
+        "time_to_converge_speedup_vs_fp64": (
+            round(results["fp64"]["time_to_converge"]["ms"] / H["time_to_converge"]["ms"], 3)
+            if "fp64" in results and head != "fp64" else None),
+        "error_vs_fp64": err_fp64,
         "poisson_1e-8": pois,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if unet is not None:
+        unet.close()
+    dev.close()
 
 
 def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args):

@@ -22,7 +22,7 @@ EXPORTS = (
     "cyx_ctx_synchronize", "cyx_ctx_set_timing", "cyx_ctx_reset_timing", "cyx_ctx_read_timing", "cyx_ham_create", "cyx_ham_set_veff", "cyx_ham_destroy", "cyx_ham_weights",
     "cyx_ham_set_tuning",
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_eigsolver_workspace_bytes", "cyx_eigsolver_pass",
-    "cyx_rayleigh_ritz", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
+    "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
     "cyx_sternheimer_apply", "cyx_sternheimer_solve",
@@ -100,6 +100,7 @@ def load():
         "cyx_eigsolver_workspace_bytes": (i64, [vp, i32, i32]),
         "cyx_eigsolver_pass": (i32, [vp, i32, vp, vp, i32, i32, dbl, dbl, dbl, pd]),
         "cyx_rayleigh_ritz": (i32, [vp, i32, vp, vp, i32, pd]),
+        "cyx_ritz_residuals": (i32, [vp, i32, vp, vp, i32, pd, pd]),
         "cyx_lanczos_bounds": (i32, [vp, pd, i32, dbl, pd, pd]),
         "cyx_density_accumulate": (i32, [vp, i32, vp, i32, pd, dbl, vp]),
         "cyx_lda_pw": (i32, [vp, i64, vp, vp, vp, vp, dbl]),

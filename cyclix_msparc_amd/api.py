@@ -19,7 +19,9 @@ HIP library.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -67,8 +69,25 @@ class ProjectorPack:
                                       nat.dptr(self.image_disp), nat.iptr(self.rc_pos), nat.dptr(self.chi))
 
 
+_LIVE_DEVICES = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_devices():
+    """Close every open context, and its handles in reverse creation order,
+    while the HIP runtime is still up (interpreter exit runs this before the
+    runtime's own teardown)."""
+    for d in list(_LIVE_DEVICES):
+        try:
+            d.close()
+        except Exception:
+            pass
+
+
 class Device:
-    """A HIP context bound to torch's current stream on ``device``."""
+    """A HIP context bound to torch's current stream on ``device``.  Handles
+    made from it register themselves; ``close()`` destroys them in reverse
+    creation order, then the context."""
 
     def __init__(self, device: int = 0):
         if torch is None or not torch.cuda.is_available():
@@ -78,6 +97,8 @@ class Device:
         h = C.c_void_p()
         nat.check(self.lib.cyx_ctx_create(device, C.byref(h)))
         self.h = h
+        self._children = []
+        _LIVE_DEVICES.add(self)
         self.torch_device = torch.device("cuda", device)
         stream = torch.cuda.current_stream(self.torch_device)
         nat.check(self.lib.cyx_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
@@ -85,7 +106,7 @@ class Device:
     def synchronize(self):
         nat.check(self.lib.cyx_ctx_synchronize(self.h))
 
-    PHASES = ("stencil", "h_apply", "kb_gather", "gram", "rotate", "wnorm", "dense_eig", "aar")
+    PHASES = ("stencil", "h_apply", "kb_gather", "gram", "rotate", "wnorm", "dense_eig", "aar", "rr_apply")
 
     def set_timing(self, on=True):
         nat.check(self.lib.cyx_ctx_set_timing(self.h, 1 if on else 0))
@@ -102,8 +123,18 @@ class Device:
             out[ph] = (ms.value, n.value)
         return out
 
+    def register(self, handle):
+        """Record a handle (an object with ``close()``) made on this context."""
+        self._children.append(weakref.ref(handle))
+
     def close(self):
         if self.h:
+            self.synchronize()
+            for ref in reversed(self._children):
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._children.clear()
             self.lib.cyx_ctx_destroy(self.h)
             self.h = None
 
@@ -146,6 +177,7 @@ class Hamiltonian:
         nat.check(self.lib.cyx_ham_create(dev.h, C.byref(self._gd), nat.dptr(self.kpt), nat.dptr(veff), nat.dptr(W),
                                           C.byref(self._proj.desc) if self._proj else None, C.byref(h)))
         self.h = h
+        self.dev.register(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -215,6 +247,22 @@ class Hamiltonian:
         nat.check(self.lib.cyx_rayleigh_ritz(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()),
                                              C.c_void_p(work.data_ptr()), ns, nat.dptr(ev)))
         return ev
+
+    def ritz_residuals(self, psi, eigval, work=None):
+        """||H psi_s - eigval_s psi_s||_W / ||psi_s||_W per state, H applied in
+        FP64 arithmetic whatever psi's storage type (FP64 residual check of
+        the pairs ``src/eigSolver.m:110-124`` returns)."""
+        _check_block(psi, self.N)
+        ns = psi.shape[0]
+        if work is None:
+            work = torch.empty_like(psi)
+        ev = np.ascontiguousarray(np.asarray(eigval, dtype=float))
+        if ev.size != ns:
+            raise ValueError("one eigenvalue per state")
+        res = np.zeros(ns)
+        nat.check(self.lib.cyx_ritz_residuals(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()),
+                                              C.c_void_p(work.data_ptr()), ns, nat.dptr(ev), nat.dptr(res)))
+        return res
 
     def density_accumulate(self, rho, psi, occ, weight=1.0):
         """rho += 2 w sum_s occ_s |psi_s|^2  (``src/electronDensity.m:13-17``)."""
@@ -327,6 +375,7 @@ class FilterPlan:
                                                   nat.dptr(b) if b is not None else None,
                                                   nat.dptr(a0) if a0 is not None else None, C.byref(h)))
         self.h = h
+        self.dev.register(self)
         self.m = int(m)
 
     def tuning(self):
@@ -386,6 +435,7 @@ class Mixer:
         nat.check(self.lib.cyx_mixer_create(dev.h, self.N, int(history), float(beta), float(omega),
                                             int(pulay_frequency), C.c_void_p(x0.data_ptr()), C.byref(h)))
         self.h = h
+        self.dev.register(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -422,6 +472,7 @@ class Poisson:
         h = C.c_void_p()
         nat.check(self.lib.cyx_poisson_create(dev.h, C.byref(self._gd), C.byref(h)))
         self.h = h
+        self.dev.register(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -39,11 +39,14 @@ def _hipcc() -> str:
 # per-file flags: the stencil's plane loop is unrolled by the queue depth
 # (13, or 13 pairs), above the compiler's default limit for a forced unroll;
 # the queue must stay in registers, which needs the full unroll
-FILE_FLAGS = {"cyx_stencil.hip": ["-mllvm", "-pragma-unroll-threshold=262144"]}
+def _file_flags(name: str):
+    if name.startswith("cyx_stencil_"):
+        return ["-mllvm", "-pragma-unroll-threshold=262144"]
+    return []
 
 
 def _compile(src: Path, obj: Path, extra):
-    cmd = [_hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(src.name, []), *extra, "-I", str(CSRC), "-c", str(src), "-o",
+    cmd = [_hipcc(), *HIPCC_FLAGS, *_file_flags(src.name), *extra, "-I", str(CSRC), "-c", str(src), "-o",
            str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -524,7 +524,9 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
     return use_dist ? hetrd_dist(st, cx, a, ns, D, E, tau, trd_ws, &trd_flag) : hetrd(st, cx, a, ns, D, E, tau);
   };
   if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
-  if ((rc = stencil_step(h, dtype, psi, nullptr, other, ns, N, 1.0, 0.0, 0.0)) != CYX_OK) return rc;
+  // H psi in FP64 arithmetic for every storage type: the FP32 operator's
+  // rounded coefficients shift the Ritz values by about eps32 ||H||
+  if ((rc = stencil_apply_fp64(h, dtype, psi, other, ns, N)) != CYX_OK) return rc;
   // Hs in full: its two triangles are averaged below, as src/eigSolver.m:108 does
   if ((rc = gram_rect(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
@@ -696,7 +698,8 @@ int cyx_ctx_set_stream(cyx_ctx_t c, void* s) {
   return CYX_OK;
 }
 
-static const char* kPhaseNames[PH_COUNT] = {"stencil", "h_apply", "kb_gather", "gram", "rotate", "wnorm", "dense_eig", "aar"};
+static const char* kPhaseNames[PH_COUNT] = {"stencil", "h_apply", "kb_gather", "gram", "rotate",
+                                            "wnorm",   "dense_eig", "aar",     "rr_apply"};
 
 static int resolve_marks(cyx_ctx_s* c) {
   if (c->marks.empty()) return CYX_OK;
@@ -927,6 +930,25 @@ int cyx_eigsolver_pass(cyx_ham_t h, int dtype, void* psi, void* work, int ns, in
   if ((rc = rr_impl(h, dtype, filt, other, ns, eigval, &res)) != CYX_OK) return rc;
   if (res != psi) CYX_HIP(hipMemcpyAsync(psi, res, N * ns * dtype_size(dtype), hipMemcpyDeviceToDevice, h->ctx->stream));
   CYX_HIP(hipStreamSynchronize(h->ctx->stream));
+  return CYX_OK;
+}
+
+int cyx_ritz_residuals(cyx_ham_t h, int dtype, const void* psi, void* work, int ns, const double* eigval,
+                       double* resid) {
+  CYX_CHECK(h && psi && work && eigval && resid && psi != work, "null or aliased argument");
+  int rc;
+  if ((rc = check_dtype(h, dtype)) != CYX_OK) return rc;
+  CYX_CHECK(ns >= 1 && ns <= h->N(), "bad state count %d", ns);
+  const int64_t N = h->N();
+  if ((rc = ensure(h->sub, std::max(h->sub.bytes, 2 * (size_t)ns * sizeof(double)))) != CYX_OK) return rc;
+  double* lam = (double*)h->sub.p;
+  double* res = lam + ns;
+  hipStream_t st = h->ctx->stream;
+  CYX_HIP(hipMemcpyAsync(lam, eigval, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((rc = stencil_apply_fp64(h, dtype, psi, work, ns, N)) != CYX_OK) return rc;
+  if ((rc = ritz_residuals(h, dtype, psi, work, lam, ns, N, res)) != CYX_OK) return rc;
+  CYX_HIP(hipMemcpyAsync(resid, res, ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  CYX_HIP(hipStreamSynchronize(st));
   return CYX_OK;
 }
 

@@ -68,7 +68,18 @@ struct DeviceBuf {
 
 namespace cyx {
 // Phases timed with HIP events on the context's stream when timing is on.
-enum Phase { PH_STENCIL = 0, PH_HAPPLY, PH_KB_GATHER, PH_GRAM, PH_ROTATE, PH_WNORM, PH_DENSE_EIG, PH_AAR, PH_COUNT };
+enum Phase {
+  PH_STENCIL = 0,
+  PH_HAPPLY,
+  PH_KB_GATHER,
+  PH_GRAM,
+  PH_ROTATE,
+  PH_WNORM,
+  PH_DENSE_EIG,
+  PH_AAR,
+  PH_RR_APPLY,
+  PH_COUNT
+};
 }
 
 struct cyx_ctx_s {
@@ -156,6 +167,10 @@ int ensure(DeviceBuf& b, size_t bytes);
 // handle has projectors (gather launched internally on `in`).
 int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void* out, int ncol, int64_t ld,
                  double alpha, double shift, double beta);
+// out = H in with FP64 arithmetic and the FP64 coefficients whatever the
+// storage type of in/out (FP32 blocks are widened on load and rounded on
+// store): the Rayleigh-Ritz and residual H-applies of the mixed path
+int stencil_apply_fp64(cyx_ham_s* h, int dtype, const void* in, void* out, int ncol, int64_t ld);
 
 // subspace kernels (FP64 accumulation)
 int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out /*dev*/);
@@ -166,6 +181,9 @@ int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const
 int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
 int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* scal /*dev ns*/);
 int hermitize(cyx_ham_s* h, int dtype, void* M, int ns);
+// res[s] = ||HA_s - lam_s A_s||_W / ||A_s||_W in FP64 (lam: device, ns)
+int ritz_residuals(cyx_ham_s* h, int dtype, const void* A, const void* HA, const double* lam, int ns, int64_t ld,
+                   double* res /*dev ns*/);
 
 // learned preconditioner: u = U(D^{-1} r) (FP32), see cyx_unet.hip
 int unet_precondition(cyx_unet_s* net, const double* coef, const double* r, float* u);

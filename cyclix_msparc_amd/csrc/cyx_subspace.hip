@@ -291,6 +291,35 @@ __global__ void __launch_bounds__(1024) colnorm_kernel(const T* __restrict__ A, 
   if (threadIdx.x == 0) scal[s] = 1.0 / sqrt(red[0]);
 }
 
+// res[s] = sqrt(sum W |ha - lam_s a|^2 / sum W |a|^2), FP64 accumulation
+template <class T>
+__global__ void __launch_bounds__(1024) resnorm_kernel(const T* __restrict__ A, const T* __restrict__ HA,
+                                                       const double* __restrict__ lam, const double* __restrict__ W,
+                                                       int64_t N, int64_t ld, double* __restrict__ res) {
+  __shared__ double red[2][1024];
+  const int s = blockIdx.x;
+  const T* a = A + (int64_t)s * ld;
+  const T* ha = HA + (int64_t)s * ld;
+  const double l = lam[s];
+  double rr = 0, aa = 0;
+  for (int64_t n = threadIdx.x; n < N; n += blockDim.x) {
+    const auto x = widen(a[n]);
+    rr += W[n] * absq(rfma(-l, x, widen(ha[n])));
+    aa += W[n] * absq(x);
+  }
+  red[0][threadIdx.x] = rr;
+  red[1][threadIdx.x] = aa;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[s] = sqrt(red[0][0] / red[1][0]);
+}
+
 template <class T>
 __global__ void colscale_kernel(T* __restrict__ A, const double* __restrict__ scal, int64_t N, int64_t ld, int ns) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -418,6 +447,29 @@ int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* sca
       return wnorm_impl<cplx<double>>(h, A, ns, ld, scal);
     case CYX_C64:
       return wnorm_impl<cplx<float>>(h, A, ns, ld, scal);
+  }
+  return CYX_ERR_ARG;
+}
+
+template <class T>
+static int resid_impl(cyx_ham_s* h, const void* A, const void* HA, const double* lam, int ns, int64_t ld, double* res) {
+  hipLaunchKernelGGL(resnorm_kernel<T>, dim3(ns), dim3(1024), 0, h->ctx->stream, (const T*)A, (const T*)HA, lam, h->W,
+                     h->N(), ld, res);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+int ritz_residuals(cyx_ham_s* h, int dtype, const void* A, const void* HA, const double* lam, int ns, int64_t ld,
+                   double* res) {
+  switch (dtype) {
+    case CYX_F64:
+      return resid_impl<double>(h, A, HA, lam, ns, ld, res);
+    case CYX_F32:
+      return resid_impl<float>(h, A, HA, lam, ns, ld, res);
+    case CYX_C128:
+      return resid_impl<cplx<double>>(h, A, HA, lam, ns, ld, res);
+    case CYX_C64:
+      return resid_impl<cplx<float>>(h, A, HA, lam, ns, ld, res);
   }
   return CYX_ERR_ARG;
 }

@@ -59,6 +59,7 @@ class UNet:
         h = C.c_void_p()
         nat.check(self.lib.cyx_unet_create(dev.h, C.byref(self._gd), C.byref(h)))
         self.h = h
+        dev.register(self)
         self.nparam = int(self.lib.cyx_unet_num_params(h))
         self.attached = []
 

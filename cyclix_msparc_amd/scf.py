@@ -176,6 +176,33 @@ class GroundState:
             h.set_veff(self.veff)
         self.psi = None
         self.work = None
+        self._psi0 = None
+        # the state the SCF loop starts from (src/scf.m:32-39), for restart()
+        self._start = tuple(t.clone() for t in (self.rho, self.phi, self.vxc, self.veff))
+
+    def restart(self):
+        """Return to the state before the first SCF iteration: the atomic
+        density and its potential, and the psi0 blocks of init_psi (device
+        copies; nothing crosses PCIe)."""
+        if self._psi0 is None:
+            raise RuntimeError("init_psi has not been called")
+        for dst, src in zip((self.rho, self.phi, self.vxc, self.veff), self._start):
+            dst.copy_(src)
+        for p, p0 in zip(self.psi, self._psi0):
+            p.copy_(p0)
+        for h in self.hams:
+            h.set_veff(self.veff)
+
+    def residuals(self):
+        """FP64 residual check of the current Ritz pairs: per channel, the
+        relative W-norm residual of every state (``cyx_ritz_residuals``)."""
+        return [h.ritz_residuals(self.psi[ks], self.EigVal[:, ks], self.work[ks]) for ks, h in enumerate(self.hams)]
+
+    def close(self):
+        """Destroy the device handles (Hamiltonians, Poisson operator)."""
+        for h in self.hams:
+            h.close()
+        self.poisson.close()
 
     def init_psi(self, psi0=None, seed=0):
         """psi blocks (Nev, N) per channel.  psi0: list of host arrays, or
@@ -200,6 +227,7 @@ class GroundState:
             p = torch.from_numpy(np.ascontiguousarray(X)).to(td).to(dt).contiguous()
             self.psi.append(p)
             self.work.append(torch.empty_like(p))
+        self._psi0 = [p.clone() for p in self.psi]
 
     def run(self, psi0=None, seed=0) -> SCFResult:
         """``src/scf.m:88-306``: iterate to SCF_tol (at least MINIT_SCF, at
@@ -233,40 +261,44 @@ class GroundState:
         max_scf = o.MAXIT_SCF
         min_scf = min(o.MINIT_SCF, max_scf)
         self.result = None
-        while (err > o.SCF_tol and count_scf <= max_scf) or count_scf <= min_scf:
-            ubgv, self.psi, EigVal, a0, bup, cut = eigSolver(S, count, ubgv, self.psi, EigVal, a0, bup, cut)
-            self.bounds = (a0.copy(), bup.copy(), cut.copy())
-            lam_f, occ = occupations(EigVal, self.wkpt, o.bet, self.nelectron, o.elec_T_type)
-            if count >= o.rhoTrigger:
-                # electron density (src/electronDensity.m:13-17)
-                self.rho.zero_()
-                for ks, h in enumerate(self.hams):
-                    h.density_accumulate(self.rho, self.psi[ks], occ[:, ks], self.wkpt[ks])
-                self.phi, info = self.poisson.solve(self.rho, self.b, self.phi, o.poisson_tol, o.MAXIT_POISSON)
-                self.vxc, veff_out = lda_pw(self.dev, self.rho, self.phi, o.rhotol)
-                h0 = self.hams[0]
-                Exc, Exc_dc, Eelec_dc = h0.energy_terms(self.rho, self.vxc, self.phi, self.b, o.rhotol)
-                Eband, Eent = band_entropy_energy(EigVal, occ, self.wkpt, lam_f, o.bet, o.elec_T_type)
-                Escc, dv2, v2 = h0.scf_terms(veff_out, veff_in, self.rho)
-                Etot = Eband + Exc - Exc_dc + Eelec_dc - self.Eself + self.Ecorr + Eent + Escc
-                err = math.sqrt(dv2) / math.sqrt(v2)
-                rec = SCFRecord(count_scf, Etot, Eband, Exc, Exc_dc, Eelec_dc, Eent, Escc, lam_f, err,
-                                int(info.iterations))
-                history.append(rec)
-                if o.verbose:
-                    print(f" SCF {count_scf:3d}  Etot = {Etot:.10f}  Eatom = {Etot / self.n_atom:.10f}  "
-                          f"err = {err:.4e}  AAR {info.iterations}", flush=True)
-                if err > o.SCF_tol or count_scf < o.MINIT_SCF:
-                    mixer.step(veff_out, veff_in, count_scf, out=veff_out)
-                    veff_in.copy_(veff_out)
-                self.veff = veff_out
-                for h in self.hams:
-                    h.set_veff(self.veff)
-                count_scf += 1
-                yield rec
-            count += 1
-        mixer.close()
+        try:
+            while (err > o.SCF_tol and count_scf <= max_scf) or count_scf <= min_scf:
+                ubgv, self.psi, EigVal, a0, bup, cut = eigSolver(S, count, ubgv, self.psi, EigVal, a0, bup, cut)
+                self.EigVal = EigVal
+                self.bounds = (a0.copy(), bup.copy(), cut.copy())
+                lam_f, occ = occupations(EigVal, self.wkpt, o.bet, self.nelectron, o.elec_T_type)
+                if count >= o.rhoTrigger:
+                    # electron density (src/electronDensity.m:13-17)
+                    self.rho.zero_()
+                    for ks, h in enumerate(self.hams):
+                        h.density_accumulate(self.rho, self.psi[ks], occ[:, ks], self.wkpt[ks])
+                    self.phi, info = self.poisson.solve(self.rho, self.b, self.phi, o.poisson_tol, o.MAXIT_POISSON)
+                    self.vxc, veff_out = lda_pw(self.dev, self.rho, self.phi, o.rhotol)
+                    h0 = self.hams[0]
+                    Exc, Exc_dc, Eelec_dc = h0.energy_terms(self.rho, self.vxc, self.phi, self.b, o.rhotol)
+                    Eband, Eent = band_entropy_energy(EigVal, occ, self.wkpt, lam_f, o.bet, o.elec_T_type)
+                    Escc, dv2, v2 = h0.scf_terms(veff_out, veff_in, self.rho)
+                    Etot = Eband + Exc - Exc_dc + Eelec_dc - self.Eself + self.Ecorr + Eent + Escc
+                    err = math.sqrt(dv2) / math.sqrt(v2)
+                    rec = SCFRecord(count_scf, Etot, Eband, Exc, Exc_dc, Eelec_dc, Eent, Escc, lam_f, err,
+                                    int(info.iterations))
+                    history.append(rec)
+                    if o.verbose:
+                        print(f" SCF {count_scf:3d}  Etot = {Etot:.10f}  Eatom = {Etot / self.n_atom:.10f}  "
+                              f"err = {err:.4e}  AAR {info.iterations}", flush=True)
+                    if err > o.SCF_tol or count_scf < o.MINIT_SCF:
+                        mixer.step(veff_out, veff_in, count_scf, out=veff_out)
+                        veff_in.copy_(veff_out)
+                    self.veff = veff_out
+                    for h in self.hams:
+                        h.set_veff(self.veff)
+                    count_scf += 1
+                    yield rec
+                count += 1
+        finally:
+            mixer.close()
         converged = not (count_scf == max_scf + 1 and err > o.SCF_tol)
         self.EigVal = EigVal
+        self.chefsi_passes = count - 1
         self.result = SCFResult(converged=converged, count_SCF=count_scf - 1, Etotal=Etot, Eatom=Etot / self.n_atom,
                                 lambda_f=lam_f, EigVal=EigVal.copy(), occ=occ, history=history)

@@ -94,9 +94,11 @@ int cyx_ctx_synchronize(cyx_ctx_t ctx);
 /* Per-phase device timers (HIP events recorded on the context's stream
  * around every launch of a phase).  Phases: "stencil" (fused H-apply +
  * three-term Chebyshev step), "h_apply" (stencil launches without the
- * three-term update: first filter step, H psi), "kb_gather", "gram",
- * "rotate", "wnorm", "dense_eig", "aar" (one whole AAR solve).  The reference prints only tic/toc wall
- * times (src/scf.m:284-304, src/poissonSolve.m:49). */
+ * three-term update: first filter step, Lanczos), "kb_gather" (KB gathers
+ * and scatters), "gram", "rotate", "wnorm", "dense_eig", "aar" (one whole
+ * AAR solve), "rr_apply" (the FP64-arithmetic H psi of Rayleigh-Ritz and of
+ * the residual check).  The reference prints only tic/toc wall times
+ * (src/scf.m:284-304, src/poissonSolve.m:49). */
 int cyx_ctx_set_timing(cyx_ctx_t ctx, int enable);
 int cyx_ctx_reset_timing(cyx_ctx_t ctx);
 int cyx_ctx_read_timing(cyx_ctx_t ctx, const char* phase, double* total_ms, int64_t* count);
@@ -151,6 +153,15 @@ int cyx_eigsolver_pass(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, 
 
 /* Rayleigh-Ritz only (src/eigSolver.m:99-124) on an already filtered psi. */
 int cyx_rayleigh_ritz(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, double* eigval);
+
+/* FP64 residual check of Ritz pairs: resid[s] = ||H psi_s - eigval[s] psi_s||_W
+ * / ||psi_s||_W with H applied in FP64 arithmetic and FP64 coefficients for
+ * every storage type (an FP32 block is widened exactly), W = S.W.  It checks
+ * the eigenpairs that eigSolver returns (src/eigSolver.m:110-124); the
+ * reference computes no residual.  psi, work: device N x ns (ld = N);
+ * eigval, resid: host, ns doubles. */
+int cyx_ritz_residuals(cyx_ham_t ham, int dtype, const void* psi, void* work, int ns, const double* eigval,
+                       double* resid);
 
 /* Extreme Ritz values of H by Lanczos from start vector v0 (host, N doubles,
  * real part used); stands in for eigs(Hfun,N,1,'lr'/'sr') at
