@@ -302,6 +302,41 @@ def nonlocal_apply(atoms, W, X, kpt):
     return out if X.ndim == 2 else out[:, 0]
 
 
+Y00 = 0.5 / np.sqrt(np.pi)  # src/sphericalHarmonics.m:64-67
+
+
+def to_cartesian(g: Grid, rtz):
+    """(r, theta, z) -> (x, y, z) with the helical angle theta + twist z.
+    ``Cychel/coordinateTransformation_cychel.m:3-7``."""
+    rtz = np.atleast_2d(np.asarray(rtz, dtype=float))
+    ang = rtz[:, 1] + g.twist * rtz[:, 2]
+    return np.stack([rtz[:, 0] * np.cos(ang), rtz[:, 0] * np.sin(ang), rtz[:, 2]], axis=1)
+
+
+def projector_tables_bruteforce(g: Grid, atoms_rtz, radial, rc, max_image=3):
+    """s-channel KB tables by brute force: for every atom and every periodic
+    image (theta shifts of L2 and z shifts of L3 up to +-max_image), every
+    grid node within Cartesian distance rc of the image, with
+    chi = radial(d) * Y00.  ``src/calculate_nloc_projector.m:23-287`` finds
+    the same sets through rb boxes around each image; this scans every node.
+    Returns {(atom, ky, kz): (sorted 0-based nodes, chi)}."""
+    ii, jj, kk = np.meshgrid(np.arange(g.nr), np.arange(g.nt), np.arange(g.nz), indexing="ij")
+    ii, jj, kk = (a.ravel(order="F") for a in (ii, jj, kk))
+    node = kk * g.nr * g.nt + jj * g.nr + ii
+    P = to_cartesian(g, np.stack([g.xin + ii * g.dr, jj * g.dt, kk * g.dz], axis=1))
+    out = {}
+    for J, x0 in enumerate(np.atleast_2d(atoms_rtz)):
+        for kz in range(-max_image, max_image + 1):
+            for ky in range(-max_image, max_image + 1):
+                c = to_cartesian(g, [x0[0], x0[1] + ky * g.L2, x0[2] + kz * g.L3])[0]
+                d = np.linalg.norm(P - c, axis=1)
+                sel = d <= rc
+                if np.any(sel):
+                    order = np.argsort(node[sel], kind="stable")
+                    out[(J, ky, kz)] = (node[sel][order], radial(d[sel][order]) * Y00)
+    return out
+
+
 @dataclass
 class Hamiltonian:
     """Everything ``h_nonlocal_vector_mult`` reads for one channel."""
@@ -724,8 +759,10 @@ def scf(g: Grid, W, atoms, channels, wkpt, bcharge, rho0, Eself, Ecorr, nelec, n
             count_scf += 1
         count += 1
     converged = not (count_scf == maxit_scf + 1 and err > SCF_tol)
+    # veff_in: the potential the last eigenpairs belong to; vxc: Vxc of rho
     return dict(converged=converged, count_SCF=count_scf - 1, Etotal=Etot, Eatom=Etot / n_atom, lambda_f=lam_f,
-                EigVal=EigVal, occ=np.stack(occs, 1), history=history, psi=psis, rho=rho, phi=phi, veff=veff)
+                EigVal=EigVal, occ=np.stack(occs, 1), history=history, psi=psis, rho=rho, phi=phi, veff=veff,
+                veff_in=veff_in, vxc=vxc)
 
 
 def bicgstab(A, b, x0, tol, maxit, Minv=None):
