@@ -360,15 +360,16 @@ def make_system(cfg: str) -> CyclixSystem:
         xout = R + 8.0
         g = GridSpec(nr, nt, nz, xin, (xout - xin) / (nr - 1), L2 / nt, L3 / nz, 0.0, 6, 3)
         return CyclixSystem(g, atoms, Species(), nstates=24, npl=16, kpt=(0.0, 0.0, 0.0), name="cnt6_cyclic")
-    if cfg in ("cnt10_helical", "cfg2"):
+    if cfg in ("cnt10_helical", "cfg2", "cnt10_helical_80x64"):
         tw = 0.05
         atoms, R, L2, L3 = zigzag_sector_atoms(10, 8, twist=tw)
-        nr, nt, nz = 120, 24, 96
+        # the second grid of the batched workload: 80 x 24 x 64 nodes over the same sector
+        nr, nt, nz = (80, 24, 64) if cfg == "cnt10_helical_80x64" else (120, 24, 96)
         xin = max(R - 8.0, 1.5)
         xout = R + 8.0
         g = GridSpec(nr, nt, nz, xin, (xout - xin) / (nr - 1), L2 / nt, L3 / nz, tw, 6, 5)
         return CyclixSystem(g, atoms, Species(), nstates=256, npl=24,
-                            kpt=(0.0, 1.0, 0.25 * 2 * math.pi / L3), name="cnt10_helical")
+                            kpt=(0.0, 1.0, 0.25 * 2 * math.pi / L3), name=cfg if cfg != "cfg2" else "cnt10_helical")
     raise ValueError(f"unknown workload {cfg!r}")
 
 
