@@ -21,7 +21,8 @@ EXPORTS = (
     "cyx_last_error", "cyx_version", "cyx_ctx_create", "cyx_ctx_destroy", "cyx_ctx_set_stream",
     "cyx_ctx_synchronize", "cyx_ctx_set_timing", "cyx_ctx_reset_timing", "cyx_ctx_read_timing", "cyx_ham_create", "cyx_ham_set_veff", "cyx_ham_destroy", "cyx_ham_weights",
     "cyx_ham_set_tuning",
-    "cyx_h_apply", "cyx_chebyshev_filter", "cyx_eigsolver_workspace_bytes", "cyx_eigsolver_pass",
+    "cyx_h_apply", "cyx_chebyshev_filter", "cyx_chebyshev_filter_noalloc", "cyx_eigsolver_workspace_bytes",
+    "cyx_eigsolver_pass",
     "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
@@ -97,6 +98,7 @@ def load():
         "cyx_ham_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "cyx_h_apply": (i32, [vp, i32, vp, vp, i32, i64]),
         "cyx_chebyshev_filter": (i32, [vp, i32, vp, vp, i32, i64, i32, dbl, dbl, dbl]),
+        "cyx_chebyshev_filter_noalloc": (i32, [vp, i32, vp, vp, i32, i64, i32, dbl, dbl, dbl]),
         "cyx_eigsolver_workspace_bytes": (i64, [vp, i32, i32]),
         "cyx_eigsolver_pass": (i32, [vp, i32, vp, vp, i32, i32, dbl, dbl, dbl, pd]),
         "cyx_rayleigh_ritz": (i32, [vp, i32, vp, vp, i32, pd]),

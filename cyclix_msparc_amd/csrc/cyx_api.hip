@@ -619,7 +619,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
 
 // Chebyshev recurrence over buffers A (holds X) and B; returns the buffer holding Y
 static int cheb_impl(cyx_ham_s* h, int dtype, void* A, void* B, int ncol, int64_t ld, int m, double a, double b,
-                     double a0, void** result) {
+                     double a0, void** result, const KbScratch* ks = nullptr) {
   CYX_CHECK(m >= 1, "Chebyshev degree must be >= 1");
   CYX_CHECK(b > a && a0 < a, "filter needs a0 < a < b (got a0=%g a=%g b=%g)", a0, a, b);
   const double e = (b - a) / 2;
@@ -630,10 +630,10 @@ static int cheb_impl(cyx_ham_s* h, int dtype, void* A, void* B, int ncol, int64_
   int rc;
   void* X = A;
   void* Y = B;
-  if ((rc = stencil_step(h, dtype, X, nullptr, Y, ncol, ld, sigma1 / e, c, 0.0)) != CYX_OK) return rc;
+  if ((rc = stencil_step(h, dtype, X, nullptr, Y, ncol, ld, sigma1 / e, c, 0.0, ks)) != CYX_OK) return rc;
   for (int i = 2; i <= m; ++i) {
     const double sigma2 = 1 / (gamma - sigma);
-    if ((rc = stencil_step(h, dtype, Y, X, X, ncol, ld, 2 * sigma2 / e, c, sigma * sigma2)) != CYX_OK) return rc;
+    if ((rc = stencil_step(h, dtype, Y, X, X, ncol, ld, 2 * sigma2 / e, c, sigma * sigma2, ks)) != CYX_OK) return rc;
     std::swap(X, Y);
     sigma = sigma2;
   }
@@ -658,6 +658,9 @@ int cyx_ctx_create(int device, cyx_ctx_t* out) {
   CYX_HIP(hipGetDeviceCount(&n));
   CYX_CHECK(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
   CYX_HIP(hipSetDevice(device));
+  // device address of the no-allocation scratch, once per device
+  int rc0;
+  if ((rc0 = noalloc_scratch_init(device)) != CYX_OK) return rc0;
   cyx_ctx_s* c = new cyx_ctx_s();
   c->device = device;
   if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -892,6 +895,31 @@ int cyx_chebyshev_filter(cyx_ham_t h, int dtype, void* X, void* Y, int ncol, int
   CYX_CHECK(ncol >= 1 && ld >= h->N(), "bad block shape");
   void* res = nullptr;
   if ((rc = cheb_impl(h, dtype, X, Y, ncol, ld, m, a, b, a0, &res)) != CYX_OK) return rc;
+  if (res != Y)
+    CYX_HIP(hipMemcpy2DAsync(Y, ld * dtype_size(dtype), res, ld * dtype_size(dtype), h->N() * dtype_size(dtype), ncol,
+                             hipMemcpyDeviceToDevice, h->ctx->stream));
+  return CYX_OK;
+}
+
+int cyx_chebyshev_filter_noalloc(cyx_ham_t h, int dtype, void* X, void* Y, int ncol, int64_t ld, int m, double a,
+                                 double b, double a0) {
+  CYX_CHECK(h && X && Y && X != Y, "null or aliased argument");
+  int rc;
+  if ((rc = check_dtype(h, dtype)) != CYX_OK) return rc;
+  CYX_CHECK(ncol >= 1 && ld >= h->N(), "bad block shape");
+  // the scratch bound is checked before anything is enqueued
+  const size_t need = kb_scratch_bytes(h, dtype, ncol);
+  CYX_CHECK(need <= NOALLOC_SCRATCH_BYTES,
+            "cyx_chebyshev_filter_noalloc: %d columns need %zu bytes of KB scratch, above the %zu-byte maximum; "
+            "use fewer columns per call or cyx_chebyshev_filter", ncol, need, NOALLOC_SCRATCH_BYTES);
+  char* base = (char*)noalloc_scratch(h->device);
+  CYX_CHECK(base != nullptr, "no scratch address for device %d (no context was created on it)", h->device);
+  // C region as kb_scratch_bytes lays it out: ncol x nproj of the block's
+  // type, rounded up to 256 bytes, then the partials
+  const size_t c_region = ((size_t)ncol * h->nproj * dtype_size(dtype) + 255) & ~(size_t)255;
+  const KbScratch ks{base, base + c_region};
+  void* res = nullptr;
+  if ((rc = cheb_impl(h, dtype, X, Y, ncol, ld, m, a, b, a0, &res, &ks)) != CYX_OK) return rc;
   if (res != Y)
     CYX_HIP(hipMemcpy2DAsync(Y, ld * dtype_size(dtype), res, ld * dtype_size(dtype), h->N() * dtype_size(dtype), ncol,
                              hipMemcpyDeviceToDevice, h->ctx->stream));

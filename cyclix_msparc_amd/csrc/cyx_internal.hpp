@@ -162,11 +162,27 @@ namespace cyx {
 
 int ensure(DeviceBuf& b, size_t bytes);
 
+// Caller-provided KB scratch of a stencil step: C (ncol x nproj of the
+// arithmetic type) and the gather partials (ncol x npair, FP64 accumulator
+// type).  Null: the handle's lazily grown buffers.
+struct KbScratch {
+  void* C;
+  void* part;
+};
+// bytes of KB scratch one step of `ncol` columns needs (C region rounded up
+// to 256 bytes, then the partials)
+size_t kb_scratch_bytes(cyx_ham_s* h, int dtype, int ncol);
+// the no-allocation entry points' device scratch: a file-scope device array
+// of NOALLOC_SCRATCH_BYTES, its address looked up once per device
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+int noalloc_scratch_init(int device);   // with `device` current
+void* noalloc_scratch(int device);      // null before init
+
 // stencil step: out = alpha*((H - shift) in) - beta*prev   (prev may be null;
 // out may alias prev).  Dispatches on dtype.  Includes the KB term when the
 // handle has projectors (gather launched internally on `in`).
 int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void* out, int ncol, int64_t ld,
-                 double alpha, double shift, double beta);
+                 double alpha, double shift, double beta, const KbScratch* ks = nullptr);
 // out = H in with FP64 arithmetic and the FP64 coefficients whatever the
 // storage type of in/out (FP32 blocks are widened on load and rounded on
 // store): the Rayleigh-Ritz and residual H-applies of the mixed path

@@ -2,6 +2,7 @@
 // step, the KB gather / scatter kernels around it, and the batched filter
 // plans (many channels per launch, captured in a HIP graph).  The stencil
 // kernels themselves are in cyx_stencil_kernel.hpp.
+#include <mutex>
 #include <type_traits>
 
 #include "cyx_internal.hpp"
@@ -336,41 +337,82 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
 }
 
 
-template <class T, class TS = T>
-static int gather_launch(cyx_ham_s* h, const void* in_, int ncol, int64_t ld, hipStream_t st) {
+// bytes of the KB scratch of one step: C, rounded up to 256 bytes, then the
+// gather partials
+template <class T>
+static size_t kb_scratch_need(cyx_ham_s* h, int ncol) {
   const KbTables<T>& K = kb_of<T>(h);
+  if (K.nproj == 0) return 0;
+  using D = typename acc_of<T>::type;
+  const size_t c = ((size_t)ncol * K.nproj * sizeof(T) + 255) & ~(size_t)255;
+  return c + (size_t)ncol * std::max(K.npair, 1) * sizeof(D);
+}
+
+size_t kb_scratch_bytes(cyx_ham_s* h, int dtype, int ncol) {
+  switch (dtype) {
+    case CYX_F64:
+      return kb_scratch_need<double>(h, ncol);
+    case CYX_F32:
+      return kb_scratch_need<float>(h, ncol);
+    case CYX_C128:
+      return kb_scratch_need<cplx<double>>(h, ncol);
+    case CYX_C64:
+      return kb_scratch_need<cplx<float>>(h, ncol);
+  }
+  return 0;
+}
+
+// KB coefficients C of `in`; ks (optional) supplies C and the partials, else
+// the handle's lazily grown buffers are used.  Returns C through *C_out.
+template <class T, class TS = T>
+static int gather_launch(cyx_ham_s* h, const void* in_, int ncol, int64_t ld, hipStream_t st, const KbScratch* ks,
+                         T** C_out) {
+  const KbTables<T>& K = kb_of<T>(h);
+  *C_out = nullptr;
   if (K.nproj == 0) return CYX_OK;
   using D = typename acc_of<T>::type;
   const int ngrp = ceil_div(ncol, GCOL);
-  CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
-  // the partials share the Gram kernels' scratch (stream-ordered uses; filter
-  // plans hold C buffers of their own, so kbC may grow here)
-  CYX_CHECK(ensure(h->small, (size_t)ncol * std::max(K.npair, 1) * sizeof(D)) == CYX_OK,
-            "kb scratch allocation failed");
-  D* part = (D*)h->small.p;
+  T* C;
+  D* part;
+  if (ks) {
+    C = (T*)ks->C;
+    part = (D*)ks->part;
+  } else {
+    CYX_CHECK(ensure(h->kbC, (size_t)ncol * K.nproj * sizeof(T)) == CYX_OK, "kb scratch allocation failed");
+    // the partials share the Gram kernels' scratch (stream-ordered uses;
+    // filter plans hold C buffers of their own, so kbC may grow here)
+    CYX_CHECK(ensure(h->small, (size_t)ncol * std::max(K.npair, 1) * sizeof(D)) == CYX_OK,
+              "kb scratch allocation failed");
+    C = (T*)h->kbC.p;
+    part = (D*)h->small.p;
+  }
+  // every partial (column, pair) is written by one block before the reduce
+  // reads it, and the reduce writes every C entry: nothing needs clearing
   if (K.npair > 0)
     hipLaunchKernelGGL((kb_gather_part_kernel<T, TS>), dim3(K.nblk_pair, ngrp), dim3(256), 0, st, (const TS*)in_, ld,
-                       ncol,
-                       K.blk_pair, K.pr_lo, K.pr_hi, K.g_node, K.g_coef, K.npair, part);
+                       ncol, K.blk_pair, K.pr_lo, K.pr_hi, K.g_node, K.g_coef, K.npair, part);
   hipLaunchKernelGGL(kb_gather_reduce_kernel<T>, dim3(ceil_div(ncol * K.nproj, 4)), dim3(256), 0, st,
-                     (const D*)part, ncol, K.nproj, K.npair, K.pp_ptr, K.gamma, (T*)h->kbC.p);
+                     (const D*)part, ncol, K.nproj, K.npair, K.pp_ptr, K.gamma, C);
   CYX_HIP(hipGetLastError());
+  *C_out = C;
   return CYX_OK;
 }
 
 // phase: timer of the stencil launch (-1: stencil / h_apply by whether prev is given)
 template <class T, class TS = T>
 static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out_, int ncol, int64_t ld, double alpha,
-                     double shift, double beta, int phase = -1) {
+                     double shift, double beta, int phase = -1, const KbScratch* ks = nullptr) {
   hipStream_t st = h->ctx->stream;
   int rc;
+  T* C = nullptr;
   {
     PhaseTimer tm(h->ctx, PH_KB_GATHER);
-    if ((rc = gather_launch<T, TS>(h, in_, ncol, ld, st)) != CYX_OK) return rc;
+    if ((rc = gather_launch<T, TS>(h, in_, ncol, ld, st, ks, &C)) != CYX_OK) return rc;
   }
   StepLaunch<T, TS> L;
   if ((rc = make_step<T, TS>(h, in_, prev_, out_, ncol, ld, alpha, shift, beta, 0, h->tune_flags, L)) != CYX_OK)
     return rc;
+  L.P.kbC = C;
   // the KB scatter runs as its own pass after the stencil: the barrier-
   // synchronised plane loop then carries no per-node table loads
   const KbTables<T>& K = kb_of<T>(h);
@@ -387,7 +429,7 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
     PhaseTimer tm(h->ctx, PH_KB_GATHER);
     hipLaunchKernelGGL((kb_scatter_kernel<T, TS>), dim3(ceil_div(K.nkbn, 256), ceil_div(ncol, KSC)), dim3(256),
                        (size_t)KSC * K.nproj * sizeof(T), st, (TS*)out_, ld, ncol, (const int*)K.kb_nodes, K.nkbn,
-                       (const uint32_t*)K.s_pk, (const T*)K.s_cph, (const int*)K.s_pj, (const T*)h->kbC.p, K.nproj,
+                       (const uint32_t*)K.s_pk, (const T*)K.s_cph, (const int*)K.s_pj, (const T*)C, K.nproj,
                        (typename traits<T>::real)alpha);
     CYX_HIP(hipGetLastError());
   }
@@ -395,19 +437,49 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
 }
 
 int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void* out, int ncol, int64_t ld,
-                 double alpha, double shift, double beta) {
+                 double alpha, double shift, double beta, const KbScratch* ks) {
   switch (dtype) {
     case CYX_F64:
-      return step_impl<double>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+      return step_impl<double>(h, in, prev, out, ncol, ld, alpha, shift, beta, -1, ks);
     case CYX_F32:
-      return step_impl<float>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+      return step_impl<float>(h, in, prev, out, ncol, ld, alpha, shift, beta, -1, ks);
     case CYX_C128:
-      return step_impl<cplx<double>>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+      return step_impl<cplx<double>>(h, in, prev, out, ncol, ld, alpha, shift, beta, -1, ks);
     case CYX_C64:
-      return step_impl<cplx<float>>(h, in, prev, out, ncol, ld, alpha, shift, beta);
+      return step_impl<cplx<float>>(h, in, prev, out, ncol, ld, alpha, shift, beta, -1, ks);
   }
   set_error("unknown dtype %d", dtype);
   return CYX_ERR_ARG;
+}
+
+// ---------------------------------------------------------------- no-allocation scratch
+// The device scratch of the no-allocation entry points (the KB coefficients
+// and gather partials of their stencil steps).  Each GPU holds its own copy
+// for the life of the process; every call on one GPU shares it, so calls
+// that overlap on one GPU race.  In host code `scratch` names a host-side
+// stand-in, never the device memory: its device address is looked up once
+// per device and only that address is used.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch_init(int device) {
+  CYX_CHECK(device >= 0 && device < NOALLOC_MAX_DEVICES, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return CYX_OK;
+  void* p = nullptr;
+  CYX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CYX_CHECK(p != nullptr, "no device address for the scratch array on device %d", device);
+  g_noalloc_ptr[device] = p;
+  return CYX_OK;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc_ptr[device];
 }
 
 int stencil_apply_fp64(cyx_ham_s* h, int dtype, const void* in, void* out, int ncol, int64_t ld) {

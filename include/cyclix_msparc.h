@@ -139,6 +139,20 @@ int cyx_h_apply(cyx_ham_t ham, int dtype, const void* X, void* HX, int ncol, int
 int cyx_chebyshev_filter(cyx_ham_t ham, int dtype, void* X, void* Y, int ncol, int64_t ld, int m, double a,
                          double b, double a0);
 
+/* cyx_chebyshev_filter without any device allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  The same kernels run;
+ * the scratch they need (the KB projection coefficients and gather partials,
+ * ncol * (nproj * sizeof(element) + npair * 16) bytes) lives in a
+ * library-owned device array of 16 MiB per GPU, held until the process ends.
+ * A call whose scratch would exceed it fails with CYX_ERR_ARG before anything
+ * is enqueued.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (from any context or thread) race; the array's contents
+ * are fully rewritten before they are read, so no call depends on the
+ * previous one.  A context must have been created on the device first. */
+int cyx_chebyshev_filter_noalloc(cyx_ham_t ham, int dtype, void* X, void* Y, int ncol, int64_t ld, int m, double a,
+                                 double b, double a0);
+
 /* Bytes of device workspace cyx_eigsolver_pass needs for `ns` states. */
 int64_t cyx_eigsolver_workspace_bytes(cyx_ham_t ham, int dtype, int ns);
 
