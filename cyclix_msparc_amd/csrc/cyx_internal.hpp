@@ -13,6 +13,22 @@ constexpr int FDN = 6;            // finite-difference half-width the kernels ar
 constexpr int QN = 2 * FDN + 1;   // z queue depth of the marching stencil
 constexpr int RCOEF = 2 * FDN + 2;  // per-radius coefficients: c+[6], c-[6], c_theta, c_diag
 
+// order-2 FDN central weights, src/initialization.m:403-419: w2 second and
+// w1 first derivative, index 0 the centre
+inline void host_fd_weights(double* w2, double* w1) {
+  double fact[2 * FDN + 2];
+  fact[0] = 1;
+  for (int k = 1; k < 2 * FDN + 2; ++k) fact[k] = fact[k - 1] * k;
+  w2[0] = 0;
+  w1[0] = 0;
+  for (int k = 1; k <= FDN; ++k) {
+    const double c = fact[FDN] * fact[FDN] / (fact[FDN - k] * fact[FDN + k]);
+    w2[k] = 2 * ((k % 2) ? 1.0 : -1.0) * c / (k * k);
+    w1[k] = ((k % 2) ? 1.0 : -1.0) * c / k;
+    w2[0] -= 2.0 / (k * k);
+  }
+}
+
 // Per-channel operator coefficients in one precision (R = double or float).
 template <class R>
 struct OpCoef {

@@ -307,6 +307,17 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.s_r1 = C.s_r1;
   P.s_z2 = C.s_z2;
   P.s_x = C.s_x;
+  {
+    // z and cross-term weights with their scale folded in, formed in FP64
+    double w2d[FDN + 1], w1d[FDN + 1];
+    const double tau = (g.cell_typ == 3) ? 0.0 : g.twist;
+    const double sz2 = -0.5 / (g.dz * g.dz), sx = (-2.0 * tau / g.dt) * (-0.5 / g.dz);
+    host_fd_weights(w2d, w1d);
+    for (int d = 0; d <= FDN; ++d) {
+      P.wz2[d] = (R)(sz2 * w2d[d]);
+      P.wx1[d] = (R)(sx * w1d[d]);
+    }
+  }
   P.xin = (R)g.xin;
   P.dr = (R)g.dr;
   P.ph_tl = phase_of<T>(h->ph[0][0], h->ph[0][1]);
@@ -323,7 +334,7 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.nproj = K.nproj;
   // LDS row stride: halo, then padding so that the next row starts on the
   // next bank group (row pitch minus tile width a multiple of 256 B)
-  P.trh = P.tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
+  P.trh = stencil_trh<T>(P.tr);
   const long nblk = (long)P.ntiles_r * P.nblocks_z * ncol;
   CYX_CHECK(nblk < (1L << 31), "grid too large");
   P.xcd_remap = (nblk % 8 == 0 && !(flags & 2)) ? 1 : 0;
@@ -594,7 +605,7 @@ int plan_setup(cyx_filter_plan_s* pl) {
               "batched channels must all be helical or all cyclic, and all with or without projectors");
     if (pl->kb && pl->kbC[c] == nullptr)
       CYX_HIP(hipMalloc(&pl->kbC[c], (size_t)pl->ncol[c] * K.nproj * sizeof(T)));
-    const size_t trh = (size_t)pl->tr + std::max(2 * FDN, (int)(256 / sizeof(T)));
+    const size_t trh = (size_t)stencil_trh<T>(pl->tr);
     shm = std::max(shm, (2 * stencil_pp<T>() * trh * (nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T));
     for (int v = 0; v < 2; ++v) {
       GatherEntry<T>& g = G[(size_t)v * nch + c];
@@ -633,7 +644,8 @@ void plan_enqueue(cyx_filter_plan_s* pl, hipStream_t st, int steps) {
     if (pl->kb)
       hipLaunchKernelGGL(kb_gather_batch_kernel<T>, dim3(pl->gblk), dim3(256), 0, st, G + ((i & 1) ? 0 : pl->nch),
                          pl->nch);
-    launch_batch<T>(E + (size_t)(i - 1) * pl->nch, pl->nch, pl->nblk, pl->nthr, pl->shm, pl->hel, i > 1, pl->kb, st);
+    launch_batch<T>(E + (size_t)(i - 1) * pl->nch, pl->nch, pl->nblk, pl->nthr, pl->shm, pl->hel, i > 1, pl->kb,
+                    stencil_nt_variant(pl->hams[0]->g.nt, pl->tr), st);
   }
 }
 

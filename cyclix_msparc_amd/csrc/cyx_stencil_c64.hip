@@ -5,5 +5,5 @@
 
 namespace cyx {
 template void launch_step<cplx<float>, cplx<float>>(const StepParams<cplx<float>, cplx<float>>&, int, int, size_t, bool, bool, bool, hipStream_t);
-template void launch_batch<cplx<float>>(const BatchEntry<cplx<float>>*, int, int, int, size_t, bool, bool, bool, hipStream_t);
+template void launch_batch<cplx<float>>(const BatchEntry<cplx<float>>*, int, int, int, size_t, bool, bool, bool, int, hipStream_t);
 }  // namespace cyx

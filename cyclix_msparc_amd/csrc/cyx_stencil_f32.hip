@@ -5,5 +5,5 @@
 
 namespace cyx {
 template void launch_step<float, float>(const StepParams<float, float>&, int, int, size_t, bool, bool, bool, hipStream_t);
-template void launch_batch<float>(const BatchEntry<float>*, int, int, int, size_t, bool, bool, bool, hipStream_t);
+template void launch_batch<float>(const BatchEntry<float>*, int, int, int, size_t, bool, bool, bool, int, hipStream_t);
 }  // namespace cyx

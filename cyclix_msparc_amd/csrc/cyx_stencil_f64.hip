@@ -5,5 +5,5 @@
 
 namespace cyx {
 template void launch_step<double, double>(const StepParams<double, double>&, int, int, size_t, bool, bool, bool, hipStream_t);
-template void launch_batch<double>(const BatchEntry<double>*, int, int, int, size_t, bool, bool, bool, hipStream_t);
+template void launch_batch<double>(const BatchEntry<double>*, int, int, int, size_t, bool, bool, bool, int, hipStream_t);
 }  // namespace cyx

@@ -47,6 +47,19 @@ template <class T>
 constexpr int stencil_pp() {
   return sizeof(T) >= 16 ? 2 : 1;
 }
+// radial tile of the compile-time-shaped variants (the heuristic tile of the
+// 120- and 80-radius grids at nt = 24 and of the 40-radius grid at nt = 16)
+constexpr int STENCIL_TR = 20;
+// LDS row stride (elements) for radial tile tr: the tile, its halo, and
+// padding so that the next row starts on the next bank group (row pitch
+// minus tile width a multiple of 256 bytes)
+template <class T>
+constexpr int stencil_trh(int tr) {
+  return tr + (2 * FDN > (int)(256 / sizeof(T)) ? 2 * FDN : (int)(256 / sizeof(T)));
+}
+// the compile-time theta count of a launch: 24 or 16 with the tile
+// STENCIL_TR, else 0 (shape read at run time)
+inline int stencil_nt_variant(int nt, int tr) { return (tr == STENCIL_TR && (nt == 24 || nt == 16)) ? nt : 0; }
 
 // T: arithmetic type of the step; TS: storage type of in/prev/out (T, or
 // the FP32 form of T for an FP64 step over an FP32 block)
@@ -68,6 +81,7 @@ struct StepParams {
   const R* veff;
   R w2[FDN + 1], w1[FDN + 1];  // FD weights (src/initialization.m:403-419)
   R s_r2, s_r1, s_z2, s_x;      // -1/2/dr^2, -1/2/dr, -1/2/dz^2, (-2 tau/dt)(-1/2/dz)
+  R wz2[FDN + 1], wx1[FDN + 1];  // s_z2 w2 and s_x w1: the z-direction and cross-term weights
   R xin, dr;
   T ph_tl, ph_tr, ph_zl, ph_zr;
   int trh;  // LDS row stride (elements): tr + halo + bank padding
@@ -116,9 +130,10 @@ template <class T, class TS>
 void launch_step(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
                  hipStream_t st);
 // Launch one batched step over several channels (T storage = arithmetic).
+// nt_variant: stencil_nt_variant of the channels' common (nt, tile).
 template <class T>
 void launch_batch(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
-                  hipStream_t st);
+                  int nt_variant, hipStream_t st);
 
 }  // namespace cyx
 
@@ -146,12 +161,20 @@ __device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
 
 // The body of one workgroup: block `bid0` of the `nblk` blocks of one
 // channel's launch (the batched kernel runs several channels in one grid).
-template <class T, class TS, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+// NT > 0: the theta count is NT and the radial tile STENCIL_TR, both known at
+// compile time, so every LDS tap is an immediate offset from one address;
+// NT = 0 reads them from P.
+template <class T, class TS, int NT, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid0, int nblk, T* lds) {
   using R = typename traits<T>::real;
   constexpr int PP = stencil_pp<T>();
-  const int TRH = P.trh;
-  const int plane_sz = TRH * (P.nt + 2 * FDN);
+  // radial taps from per-thread merged coefficients (FP32 arithmetic: 12
+  // registers; FP64 keeps the shared-weight form to save 24 registers)
+  constexpr bool MERGED = sizeof(R) == 4;
+  const int nt_ = NT > 0 ? NT : P.nt;
+  const int tr_ = NT > 0 ? STENCIL_TR : P.tr;
+  const int TRH = NT > 0 ? stencil_trh<T>(STENCIL_TR) : P.trh;
+  const int plane_sz = TRH * (nt_ + 2 * FDN);
   T* Dsh = lds + 2 * PP * plane_sz;  // per-column KB table D[il]
 
   int bid = bid0;
@@ -176,14 +199,14 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
   }
 
   const int tid = threadIdx.x;
-  const int i = tid % P.tr;
-  const int j = tid / P.tr;
-  const int r0 = tile * P.tr;
+  const int i = tid % tr_;
+  const int j = tid / tr_;
+  const int r0 = tile * tr_;
   const int r = r0 + i;
   const bool rvalid = r < P.nr;
   const int z0 = zblk * P.zb;
   const int z1 = min(z0 + P.zb, P.nz);
-  const int nrt = P.nr * P.nt;
+  const int nrt = P.nr * nt_;
   const int64_t colbase = (int64_t)s * P.ld;
   const TS* __restrict__ src = P.in + colbase;
   const int jr = j * P.nr + r;  // in-plane node offset
@@ -195,6 +218,13 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
     rfac = P.s_r1 / (P.xin + R(r) * P.dr);
     cth = P.rcoef[r * RCOEF + 2 * FDN];
     cdg = P.rcoef[r * RCOEF + 2 * FDN + 1] - P.shift;
+  }
+  // merged radial coefficients: s_r2 w2_p +- rfac w1_p for the taps r +- p
+  R cpl[FDN], cmi[FDN];
+#pragma unroll
+  for (int p = 1; p <= FDN; ++p) {
+    cpl[p - 1] = rfma(rfac, P.w1[p], P.s_r2 * P.w2[p]);
+    cmi[p - 1] = rfma(-rfac, P.w1[p], P.s_r2 * P.w2[p]);
   }
 
   if constexpr (HAS_KB) {
@@ -226,7 +256,7 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
     xa = (ra >= 0 && ra < P.nr) ? to_arith<T>(row[ra]) : zero<T>();
     xb = zero<T>();
     if (i < 2 * FDN) {
-      const int rb = r0 - FDN + P.tr + i;
+      const int rb = r0 - FDN + tr_ + i;
       if (rb < P.nr) xb = to_arith<T>(row[rb]);
     }
     if constexpr (traits<T>::is_complex) {
@@ -272,16 +302,16 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
         {
           const int row = (j + FDN) * TRH;
           L[row + i] = xa;
-          if (i < 2 * FDN) L[row + P.tr + i] = xb;
+          if (i < 2 * FDN) L[row + tr_ + i] = xb;
           if (j < FDN) {
-            const int rw = (j + FDN + P.nt) * TRH;
+            const int rw = (j + FDN + nt_) * TRH;
             L[rw + i] = cmul(P.ph_tr, xa);
-            if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tr, xb);
+            if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tr, xb);
           }
-          if (j >= P.nt - FDN) {
-            const int rw = (j + FDN - P.nt) * TRH;
+          if (j >= nt_ - FDN) {
+            const int rw = (j + FDN - nt_) * TRH;
             L[rw + i] = cmul(P.ph_tl, xa);
-            if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tl, xb);
+            if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tl, xb);
           }
         }
         const R vcur = ve;
@@ -337,35 +367,39 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
           if constexpr (HELICAL) tt = rfma(P.w1[p], sub(xp, xm), tt);
         }
         if (inner) {
-          T sr = zero<T>();
-          T dr = zero<T>();
+          T self = rfma(cth, st, rmul(cdg + vcur, x0));
+          if constexpr (MERGED) {
 #pragma unroll
-          for (int p = 1; p <= FDN; ++p) {
-            const T xp = L[c + p];
-            const T xm = L[c - p];
-            sr = rfma(P.w2[p], add(xp, xm), sr);
-            dr = rfma(P.w1[p], sub(xp, xm), dr);
+            for (int p = 1; p <= FDN; ++p) {
+              self = rfma(cpl[p - 1], L[c + p], self);
+              self = rfma(cmi[p - 1], L[c - p], self);
+            }
+          } else {
+            T sr = zero<T>();
+            T dr = zero<T>();
+#pragma unroll
+            for (int p = 1; p <= FDN; ++p) {
+              const T xp = L[c + p];
+              const T xm = L[c - p];
+              sr = rfma(P.w2[p], add(xp, xm), sr);
+              dr = rfma(P.w1[p], sub(xp, xm), dr);
+            }
+            self = rfma(P.s_r2, sr, self);
+            self = rfma(rfac, dr, self);
           }
-          T self = rfma(P.s_r2, sr, rmul(cdg + vcur, x0));
-          self = rfma(rfac, dr, self);
-          self = rfma(cth, st, self);
           q[u % QN] = add(q[u % QN], self);
         }
-        {
-          const T xz = rmul(P.s_z2, x0);
-          T tz = zero<T>();
-          if constexpr (HELICAL) tz = rmul(P.s_x, tt);
 #pragma unroll
-          for (int d = 1; d <= FDN; ++d) {
-            // plane kk feeds output planes kk-d and kk+d
-            T& qa = q[(u - d + QN) % QN];
-            T& qb = q[(u + d) % QN];
-            qa = rfma(P.w2[d], xz, qa);
-            qb = rfma(P.w2[d], xz, qb);
-            if constexpr (HELICAL) {
-              qa = rfma(P.w1[d], tz, qa);
-              qb = rfma(-P.w1[d], tz, qb);
-            }
+        for (int d = 1; d <= FDN; ++d) {
+          // plane kk feeds output planes kk-d and kk+d (z weights carry
+          // -1/2/dz^2 and the helical cross factor)
+          T& qa = q[(u - d + QN) % QN];
+          T& qb = q[(u + d) % QN];
+          qa = rfma(P.wz2[d], x0, qa);
+          qb = rfma(P.wz2[d], x0, qb);
+          if constexpr (HELICAL) {
+            qa = rfma(P.wx1[d], tt, qa);
+            qb = rfma(-P.wx1[d], tt, qb);
           }
         }
         // ---- output plane kk-FDN is complete
@@ -394,16 +428,16 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
   auto stage = [&](T* L, const T& xa, const T& xb) {
     const int row = (j + FDN) * TRH;
     L[row + i] = xa;
-    if (i < 2 * FDN) L[row + P.tr + i] = xb;
+    if (i < 2 * FDN) L[row + tr_ + i] = xb;
     if (j < FDN) {
-      const int rw = (j + FDN + P.nt) * TRH;
+      const int rw = (j + FDN + nt_) * TRH;
       L[rw + i] = cmul(P.ph_tr, xa);
-      if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tr, xb);
+      if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tr, xb);
     }
-    if (j >= P.nt - FDN) {
-      const int rw = (j + FDN - P.nt) * TRH;
+    if (j >= nt_ - FDN) {
+      const int rw = (j + FDN - nt_) * TRH;
       L[rw + i] = cmul(P.ph_tl, xa);
-      if (i < 2 * FDN) L[rw + P.tr + i] = cmul(P.ph_tl, xb);
+      if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tl, xb);
     }
   };
   // KB pipeline, per plane slot p of a pair (planes t0+p): hold[p] is the
@@ -521,20 +555,15 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
               self = rfma(cth, st, self);
               q[u % QN] = add(q[u % QN], self);
             }
-            {
-              const T xz = rmul(P.s_z2, x0);
-              T tz = zero<T>();
-              if constexpr (HELICAL) tz = rmul(P.s_x, tt);
 #pragma unroll
-              for (int d = 1; d <= FDN; ++d) {
-                T& qa = q[(u - d + QN) % QN];
-                T& qb = q[(u + d) % QN];
-                qa = rfma(P.w2[d], xz, qa);
-                qb = rfma(P.w2[d], xz, qb);
-                if constexpr (HELICAL) {
-                  qa = rfma(P.w1[d], tz, qa);
-                  qb = rfma(-P.w1[d], tz, qb);
-                }
+            for (int d = 1; d <= FDN; ++d) {
+              T& qa = q[(u - d + QN) % QN];
+              T& qb = q[(u + d) % QN];
+              qa = rfma(P.wz2[d], x0, qa);
+              qb = rfma(P.wz2[d], x0, qb);
+              if constexpr (HELICAL) {
+                qa = rfma(P.wx1[d], tt, qa);
+                qb = rfma(-P.wx1[d], tt, qb);
               }
             }
             // output plane kk-FDN is complete; emit it before the next plane
@@ -574,22 +603,22 @@ constexpr int stencil_waves_per_eu() {
   return 2 * stencil_min_blocks<T>();
 }
 
-template <class T, class TS, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+template <class T, class TS, int NT, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __global__ void __launch_bounds__(stencil_max_threads<T>())
     __attribute__((amdgpu_waves_per_eu(stencil_waves_per_eu<T>()))) stencil_step_kernel(const StepParams<T, TS> P) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  stencil_body<T, TS, HELICAL, HAS_PREV, HAS_KB>(P, blockIdx.x, gridDim.x, reinterpret_cast<T*>(smem_raw));
+  stencil_body<T, TS, NT, HELICAL, HAS_PREV, HAS_KB>(P, blockIdx.x, gridDim.x, reinterpret_cast<T*>(smem_raw));
 }
 
-template <class T, bool HELICAL, bool HAS_PREV, bool HAS_KB>
+template <class T, int NT, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __global__ void __launch_bounds__(stencil_max_threads<T>())
     __attribute__((amdgpu_waves_per_eu(stencil_waves_per_eu<T>())))
     stencil_batch_kernel(const BatchEntry<T>* __restrict__ E, int nch) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   int c = 0;
   while (c + 1 < nch && E[c + 1].blk0 <= (int)blockIdx.x) ++c;
-  stencil_body<T, T, HELICAL, HAS_PREV, HAS_KB>(E[c].P, blockIdx.x - E[c].blk0, E[c].nblk,
-                                             reinterpret_cast<T*>(smem_raw));
+  stencil_body<T, T, NT, HELICAL, HAS_PREV, HAS_KB>(E[c].P, blockIdx.x - E[c].blk0, E[c].nblk,
+                                                 reinterpret_cast<T*>(smem_raw));
 }
 
 // dynamic LDS above 64 KB (two-plane staging of complex128) must be allowed
@@ -598,51 +627,56 @@ inline void allow_lds(const void* fn, size_t shm) {
   if (shm > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
 }
 
-template <class T, class TS, bool HEL, bool PREV, bool KB>
+template <class T, class TS, int NT, bool HEL, bool PREV, bool KB>
 void launch_variant(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, hipStream_t st) {
-  allow_lds((const void*)&stencil_step_kernel<T, TS, HEL, PREV, KB>, shm);
-  hipLaunchKernelGGL((stencil_step_kernel<T, TS, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
+  allow_lds((const void*)&stencil_step_kernel<T, TS, NT, HEL, PREV, KB>, shm);
+  hipLaunchKernelGGL((stencil_step_kernel<T, TS, NT, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, P);
+}
+
+// the helical / prev choices of one compile-time shape NT (widening steps
+// have no prev operand)
+template <class T, class TS, int NT, bool KB>
+void launch_shape(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, bool hel, bool prev, hipStream_t st) {
+  if constexpr (std::is_same_v<T, TS>) {
+    if (hel) {
+      if (prev) launch_variant<T, TS, NT, true, true, KB>(P, nblk, nthr, shm, st);
+      else launch_variant<T, TS, NT, true, false, KB>(P, nblk, nthr, shm, st);
+    } else {
+      if (prev) launch_variant<T, TS, NT, false, true, KB>(P, nblk, nthr, shm, st);
+      else launch_variant<T, TS, NT, false, false, KB>(P, nblk, nthr, shm, st);
+    }
+  } else {
+    (void)prev;
+    if (hel) launch_variant<T, TS, NT, true, false, KB>(P, nblk, nthr, shm, st);
+    else launch_variant<T, TS, NT, false, false, KB>(P, nblk, nthr, shm, st);
+  }
 }
 
 template <class T, class TS>
 void launch_step(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
                  hipStream_t st) {
-#define CYX_LAUNCH(HEL, PRV, KBV) launch_variant<T, TS, HEL, PRV, KBV>(P, nblk, nthr, shm, st)
-  if constexpr (std::is_same_v<T, TS>) {
-    if (hel) {
-      if (prev) {
-        if (kb) CYX_LAUNCH(true, true, true); else CYX_LAUNCH(true, true, false);
-      } else {
-        if (kb) CYX_LAUNCH(true, false, true); else CYX_LAUNCH(true, false, false);
-      }
-    } else {
-      if (prev) {
-        if (kb) CYX_LAUNCH(false, true, true); else CYX_LAUNCH(false, true, false);
-      } else {
-        if (kb) CYX_LAUNCH(false, false, true); else CYX_LAUNCH(false, false, false);
-      }
-    }
-  } else {
-    (void)prev;  // widening steps are plain H-applies
-    if (hel) {
-      if (kb) CYX_LAUNCH(true, false, true); else CYX_LAUNCH(true, false, false);
-    } else {
-      if (kb) CYX_LAUNCH(false, false, true); else CYX_LAUNCH(false, false, false);
-    }
+  // the KB term inside the plane loop exists in the run-time-shaped variant only
+  if (kb) return launch_shape<T, TS, 0, true>(P, nblk, nthr, shm, hel, prev, st);
+  switch (stencil_nt_variant(P.nt, P.tr)) {
+    case 24:
+      return launch_shape<T, TS, 24, false>(P, nblk, nthr, shm, hel, prev, st);
+    case 16:
+      return launch_shape<T, TS, 16, false>(P, nblk, nthr, shm, hel, prev, st);
+    default:
+      return launch_shape<T, TS, 0, false>(P, nblk, nthr, shm, hel, prev, st);
   }
-#undef CYX_LAUNCH
 }
 
-template <class T, bool HEL, bool PREV, bool KB>
+template <class T, int NT, bool HEL, bool PREV, bool KB>
 void launch_batch_variant(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, hipStream_t st) {
-  allow_lds((const void*)&stencil_batch_kernel<T, HEL, PREV, KB>, shm);
-  hipLaunchKernelGGL((stencil_batch_kernel<T, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, E, nch);
+  allow_lds((const void*)&stencil_batch_kernel<T, NT, HEL, PREV, KB>, shm);
+  hipLaunchKernelGGL((stencil_batch_kernel<T, NT, HEL, PREV, KB>), dim3(nblk), dim3(nthr), shm, st, E, nch);
 }
 
-template <class T>
-void launch_batch(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
-                  hipStream_t st) {
-#define CYX_BL(HEL, PRV, KBV) launch_batch_variant<T, HEL, PRV, KBV>(E, nch, nblk, nthr, shm, st)
+template <class T, int NT>
+void launch_batch_shape(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
+                        hipStream_t st) {
+#define CYX_BL(HEL, PRV, KBV) launch_batch_variant<T, NT, HEL, PRV, KBV>(E, nch, nblk, nthr, shm, st)
   if (hel) {
     if (prev) {
       if (kb) CYX_BL(true, true, true); else CYX_BL(true, true, false);
@@ -657,6 +691,13 @@ void launch_batch(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t sh
     }
   }
 #undef CYX_BL
+}
+
+template <class T>
+void launch_batch(const BatchEntry<T>* E, int nch, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
+                  int nt_variant, hipStream_t st) {
+  if (nt_variant == 24) return launch_batch_shape<T, 24>(E, nch, nblk, nthr, shm, hel, prev, kb, st);
+  return launch_batch_shape<T, 0>(E, nch, nblk, nthr, shm, hel, prev, kb, st);
 }
 
 }  // namespace cyx
