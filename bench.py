@@ -145,6 +145,7 @@ def run_path(dev, sysm, args, precision, unet=None):
     torch.cuda.synchronize()
     ttc_ms = (time.perf_counter() - t) * 1e3
     resid = gs.residuals()[0]
+    rot32, rot64 = gs.hams[0].rotation_counts()
     nocc = int(round(sysm.nelectron / 2))
     # roofline of the dominant kernel, the fused local stencil + three-term step:
     # algorithmic bytes per launch = read Y + read X_prev + write Y_new (ns
@@ -178,6 +179,8 @@ def run_path(dev, sysm, args, precision, unet=None):
         "filter": {"ms_per_pass": round(filt_ms_pass, 3), "GB/s": round(filter_gbs, 1),
                    "frac_hbm": round(filter_gbs / HBM_PEAK_GBS, 4), "passes": n_pass},
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in tm.items()},
+        "subspace_products": ({"fp32_matrix_cores": rot32, "fp64_matrix_cores": rot64}
+                              if precision != "fp64" else "fp64_matrix_cores"),
         "scf": {"err": [round(r.err, 6) for r in recs], "Eatom": [round(r.Etotal / sysm.n_atm, 8) for r in recs],
                 "aar_iterations": [r.aar_iterations for r in recs], "restarts": restarts,
                 "post_convergence_steps": 0},

@@ -19,7 +19,7 @@ DTYPE_NAMES = {"f64": CYX_F64, "c128": CYX_C128, "f32": CYX_F32, "c64": CYX_C64}
 # every symbol declared in include/cyclix_msparc.h
 EXPORTS = (
     "cyx_last_error", "cyx_version", "cyx_ctx_create", "cyx_ctx_destroy", "cyx_ctx_set_stream",
-    "cyx_ctx_synchronize", "cyx_ctx_set_timing", "cyx_ctx_reset_timing", "cyx_ctx_read_timing", "cyx_ham_create", "cyx_ham_set_veff", "cyx_ham_destroy", "cyx_ham_weights",
+    "cyx_ctx_synchronize", "cyx_ctx_set_timing", "cyx_ctx_reset_timing", "cyx_ctx_read_timing", "cyx_ham_create", "cyx_ham_set_veff", "cyx_ham_destroy", "cyx_ham_weights", "cyx_ham_rotation_counts",
     "cyx_ham_set_tuning",
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_chebyshev_filter_noalloc", "cyx_eigsolver_workspace_bytes",
     "cyx_eigsolver_pass",
@@ -95,6 +95,7 @@ def load():
         "cyx_ham_set_veff": (i32, [vp, vp, i32]),
         "cyx_ham_destroy": (i32, [vp]),
         "cyx_ham_weights": (vp, [vp]),
+        "cyx_ham_rotation_counts": (i32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "cyx_ham_set_tuning": (i32, [vp, C.POINTER(Tuning)]),
         "cyx_h_apply": (i32, [vp, i32, vp, vp, i32, i64]),
         "cyx_chebyshev_filter": (i32, [vp, i32, vp, vp, i32, i64, i32, dbl, dbl, dbl]),

@@ -248,6 +248,12 @@ class Hamiltonian:
                                              C.c_void_p(work.data_ptr()), ns, nat.dptr(ev)))
         return ev
 
+    def rotation_counts(self):
+        """(FP32, FP64) matrix-core Rayleigh-Ritz rotations of FP32 blocks so far."""
+        a, b = C.c_int64(), C.c_int64()
+        nat.check(self.lib.cyx_ham_rotation_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def ritz_residuals(self, psi, eigval, work=None):
         """||H psi_s - eigval_s psi_s||_W / ||psi_s||_W per state, H applied in
         FP64 arithmetic whatever psi's storage type (FP64 residual check of

@@ -506,7 +506,14 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   const bool use_dist = dist_fits && (trd_req.empty() || trd_req == "dist");
   const bool own_trd = use_dist || (one_fits && trd_req != "lapack");
   const size_t dist_ws = use_dist ? hetrd_dist_ws_bytes(ns, cx) : 0;
-  const size_t need = 3 * mat + 4 * (size_t)ns * sizeof(double) + (size_t)ns * esz + 512 + dist_ws + 256;
+  // FP32 blocks: Gram and projection on the FP32 matrix cores (FP64 chunk
+  // sums), rotation there too when the overlap is well conditioned
+  // (CYX_SUBSPACE = fp64 keeps the FP64 matrix-core products; read per call)
+  const char* sub_env = getenv("CYX_SUBSPACE");
+  const bool fp32_block = dtype == CYX_F32 || dtype == CYX_C64;
+  const bool sub32 = fp32_block && !(sub_env && std::string(sub_env) == "fp64");
+  const size_t need =
+      3 * mat + 4 * (size_t)ns * sizeof(double) + (size_t)ns * esz + 512 + dist_ws + 256 + ((size_t)ns + 32) * 8;
   int rc;
   if ((rc = ensure(h->sub, need)) != CYX_OK) return rc;
   char* base = (char*)h->sub.p;
@@ -520,16 +527,58 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
   rocblas_int* info = (rocblas_int*)((char*)tau + (size_t)ns * esz + 64);
   void* trd_ws = (void*)(((uintptr_t)(info + 16) + 255) & ~(uintptr_t)255);
   unsigned* trd_flag = nullptr;
+  // saved diagonal of M and the Cholesky conditioning ratio (after the tridiagonal workspace)
+  double* mdiag = (double*)(((uintptr_t)trd_ws + dist_ws + 255) & ~(uintptr_t)255);
+  double* chol_ratio_dev = mdiag + ns;
   auto reduce = [&](void* a) -> int {
     return use_dist ? hetrd_dist(st, cx, a, ns, D, E, tau, trd_ws, &trd_flag) : hetrd(st, cx, a, ns, D, E, tau);
   };
-  if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
+  // FP32 blocks: the overlap first, on the FP32 matrix cores; its Cholesky
+  // factor either certifies the block well conditioned (min_i R_ii^2 / M_ii
+  // >= 1e-2: every column keeps a tenth of its norm outside the span of the
+  // earlier ones), and the projection and the rotation stay on the FP32
+  // cores, or not, and all three products run on the FP64 cores, whose
+  // overlap of the stored block is exact to FP64 rounding
+  bool use32 = false, factored = false;
+  if (sub32) {
+    rocblas_handle bh = (rocblas_handle)h->ctx->blas;
+    if ((rc = gram_rect32(h, dtype, psi, N, ns, psi, N, ns, h->Wm, M, true)) != CYX_OK) return rc;
+    if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
+    if ((rc = overlap_diag(h, cx, M, ns, mdiag)) != CYX_OK) return rc;
+    rocblas_status s0;
+    {
+      PhaseTimer tm(h->ctx, PH_DENSE_EIG);
+      s0 = cx ? rocsolver_zpotrf(bh, rocblas_fill_upper, ns, (rocblas_double_complex*)M, ns, info)
+              : rocsolver_dpotrf(bh, rocblas_fill_upper, ns, (double*)M, ns, info);
+    }
+    if (s0 != rocblas_status_success) {
+      set_error("overlap Cholesky failed with rocSOLVER status %d", (int)s0);
+      return CYX_ERR_SOLVER;
+    }
+    if ((rc = cholesky_ratio(h, cx, M, mdiag, ns, chol_ratio_dev)) != CYX_OK) return rc;
+    double ratio = 0.0;
+    rocblas_int pinfo = 0;
+    CYX_HIP(hipMemcpyAsync(&ratio, chol_ratio_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    CYX_HIP(hipMemcpyAsync(&pinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+    CYX_HIP(hipStreamSynchronize(st));
+    use32 = pinfo == 0 && ratio >= 1e-2;
+    factored = use32;
+    h->rot32_count += use32 ? 1 : 0;
+    h->rot64_count += use32 ? 0 : 1;
+  }
+  if (!use32)
+    if ((rc = gram_w(h, dtype, psi, psi, ns, N, M)) != CYX_OK) return rc;
   // H psi in FP64 arithmetic for every storage type: the FP32 operator's
   // rounded coefficients shift the Ritz values by about eps32 ||H||
   if ((rc = stencil_apply_fp64(h, dtype, psi, other, ns, N)) != CYX_OK) return rc;
   // Hs in full: its two triangles are averaged below, as src/eigSolver.m:108 does
-  if ((rc = gram_rect(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
-  if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
+  if (use32) {
+    if ((rc = gram_rect32(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
+  } else {
+    if ((rc = gram_rect(h, dtype, psi, N, ns, other, N, ns, h->Wm, Hs, false)) != CYX_OK) return rc;
+  }
+  if (!factored)
+    if ((rc = hermitize(h, dtype, M, ns)) != CYX_OK) return rc;
   if ((rc = hermitize(h, dtype, Hs, ns)) != CYX_OK) return rc;
   void* Q = Hs;  // eigenvectors of the pencil
   {
@@ -542,7 +591,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
       const rocblas_double_complex one = {1.0, 0.0};
       auto* m = (rocblas_double_complex*)M;
       auto* a = (rocblas_double_complex*)Hs;
-      bs[0] = rocsolver_zpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
+      bs[0] = factored ? rocblas_status_success : rocsolver_zpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
       bs[1] = rocblas_ztrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
       bs[2] = rocblas_ztrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
@@ -566,7 +615,7 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
       const double one = 1.0;
       auto* m = (double*)M;
       auto* a = (double*)Hs;
-      bs[0] = rocsolver_dpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
+      bs[0] = factored ? rocblas_status_success : rocsolver_dpotrf(bh, rocblas_fill_upper, ns, m, ns, info);
       bs[1] = rocblas_dtrsm(bh, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
                             rocblas_diagonal_non_unit, ns, ns, &one, m, ns, a, ns);
       bs[2] = rocblas_dtrsm(bh, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
@@ -592,7 +641,13 @@ static int rr_impl(cyx_ham_s* h, int dtype, void* psi, void* other, int ns, doub
         return CYX_ERR_SOLVER;
       }
   }
-  if ((rc = rotate(h, dtype, psi, Q, other, ns, N)) != CYX_OK) return rc;
+  // the FP32 rotation's error grows like eps32 / sqrt(min_i R_ii^2 / M_ii):
+  // it runs for the blocks the overlap certified above
+  if (use32) {
+    if ((rc = rotate32(h, dtype, psi, Q, other, ns, N)) != CYX_OK) return rc;
+  } else {
+    if ((rc = rotate(h, dtype, psi, Q, other, ns, N)) != CYX_OK) return rc;
+  }
   if ((rc = wnormalize(h, dtype, other, ns, N, scal)) != CYX_OK) return rc;
   rocblas_int hinfo[2] = {0, 0};
   unsigned trd_failed = 0;
@@ -867,6 +922,13 @@ int cyx_ham_destroy(cyx_ham_t h) {
 }
 
 const double* cyx_ham_weights(cyx_ham_t h) { return h ? h->W : nullptr; }
+
+int cyx_ham_rotation_counts(cyx_ham_t h, int64_t* fp32, int64_t* fp64) {
+  CYX_CHECK(h && fp32 && fp64, "null argument");
+  *fp32 = h->rot32_count;
+  *fp64 = h->rot64_count;
+  return CYX_OK;
+}
 
 int cyx_ham_set_tuning(cyx_ham_t h, const cyx_tuning* t) {
   CYX_CHECK(h && t, "null argument");

@@ -153,6 +153,8 @@ struct cyx_ham_s {
   int nproj = 0;
   // stencil launch shape (0: heuristic), see cyx_tuning
   int tune_tr = 0, tune_zb = 0, tune_flags = 0;
+  // Rayleigh-Ritz rotations of FP32 blocks on the FP32 / FP64 matrix cores
+  int64_t rot32_count = 0, rot64_count = 0;
   // lazily grown scratch
   cyx::DeviceBuf kbC;      // C coefficients (ncol x nproj x 16 B)
   cyx::DeviceBuf small;    // Gram partial sums
@@ -211,6 +213,16 @@ int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_
 int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
               const double* W, void* M_out /*dev*/, bool herm = false);
 int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
+// the same products of FP32 blocks on the FP32 matrix cores (cyx_subspace32.hip):
+// the Gram with FP32 sums over 128-point chunks added in FP64; the rotation
+// with Q rounded to FP32 and FP32 sums (for well-conditioned blocks only)
+int gram_rect32(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                const double* W, void* M_out /*dev*/, bool herm);
+int rotate32(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
+// out[i] = Re M_ii (device); ratio = min_i R_ii^2 / M_ii for the Cholesky
+// factor R stored over M (device scalar)
+int overlap_diag(cyx_ham_s* h, bool cx, const void* M, int ns, double* out);
+int cholesky_ratio(cyx_ham_s* h, bool cx, const void* R, const double* mdiag, int ns, double* out);
 int wnormalize(cyx_ham_s* h, int dtype, void* A, int ns, int64_t ld, double* scal /*dev ns*/);
 int hermitize(cyx_ham_s* h, int dtype, void* M, int ns);
 // res[s] = ||HA_s - lam_s A_s||_W / ||A_s||_W in FP64 (lam: device, ns)

@@ -48,11 +48,44 @@ struct GramStep {
   double w[SP];
 };
 
+// a lane's SP consecutive points of one column (32 bytes) as two 16-byte loads
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+template <class T, int SP>
+__device__ __forceinline__ void gram_run(T (&dst)[SP], const T* __restrict__ p) {
+  static_assert(sizeof(T) * SP == 32, "a run is 32 bytes");
+  const v4u* q = reinterpret_cast<const v4u*>(p);
+  const v4u u0 = q[0], u1 = q[1];
+  const unsigned w[8] = {u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
+  __builtin_memcpy(dst, w, 32);
+}
+
+// vec: the leading dimensions keep every lane's run 16-byte aligned
 template <class T, int SP>
 __device__ __forceinline__ void gram_load(GramStep<T, SP>& S, const T* __restrict__ A, int64_t lda, int na,
                                           const T* __restrict__ B, int64_t ldb, int nb,
                                           const double* __restrict__ W, int64_t n0, int64_t k1, const int (&ca)[2],
-                                          const int (&cb)[2], int g) {
+                                          const int (&cb)[2], int g, bool vec) {
+  const int64_t nl = n0 + SP * g;
+  if (vec && nl + SP <= k1) {
+#pragma unroll
+    for (int s = 0; s < SP; ++s) S.w[s] = W[nl + s];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (ca[i] < na) {
+        gram_run(S.a[i], A + (int64_t)ca[i] * lda + nl);
+      } else {
+#pragma unroll
+        for (int s = 0; s < SP; ++s) S.a[i][s] = zero<T>();
+      }
+      if (cb[i] < nb) {
+        gram_run(S.b[i], B + (int64_t)cb[i] * ldb + nl);
+      } else {
+#pragma unroll
+        for (int s = 0; s < SP; ++s) S.b[i][s] = zero<T>();
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < SP; ++s) {
     const int64_t n = n0 + SP * g + s;
@@ -112,7 +145,8 @@ template <class T>
 __global__ void __launch_bounds__(256, 2) gram_mfma_kernel(const T* __restrict__ A, int64_t lda, int na,
                                                         const T* __restrict__ B, int64_t ldb, int nb,
                                                         const double* __restrict__ W, int64_t N, int64_t chunk,
-                                                        int herm, int nta, typename acc_of<T>::type* __restrict__ part) {
+                                                        int herm, int nta, int vec,
+                                                        typename acc_of<T>::type* __restrict__ part) {
   using D = typename acc_of<T>::type;
   constexpr bool CX = traits<T>::is_complex;
   constexpr int NC = CX ? 2 : 1;  // accumulator sets: (re, im) or one
@@ -133,17 +167,17 @@ __global__ void __launch_bounds__(256, 2) gram_mfma_kernel(const T* __restrict__
   const int ca[2] = {a0 + c, a0 + 16 + c}, cb[2] = {b0 + c, b0 + 16 + c};
   if (k0 < k1) {
     GramStep<T, SP> S0, S1;
-    gram_load(S0, A, lda, na, B, ldb, nb, W, k0, k1, ca, cb, g);
+    gram_load(S0, A, lda, na, B, ldb, nb, W, k0, k1, ca, cb, g, vec != 0);
     for (int64_t n0 = k0;;) {
       const bool more = n0 + 4 * SP < k1;
-      if (more) gram_load(S1, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g);
+      if (more) gram_load(S1, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g, vec != 0);
       __builtin_amdgcn_sched_barrier(0);
       gram_mfma(S0, acc);
       __builtin_amdgcn_sched_barrier(0);
       if (!more) break;
       n0 += 4 * SP;
       const bool more2 = n0 + 4 * SP < k1;
-      if (more2) gram_load(S0, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g);
+      if (more2) gram_load(S0, A, lda, na, B, ldb, nb, W, n0 + 4 * SP, k1, ca, cb, g, vec != 0);
       __builtin_amdgcn_sched_barrier(0);
       gram_mfma(S1, acc);
       __builtin_amdgcn_sched_barrier(0);
@@ -370,8 +404,11 @@ static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const voi
   CYX_CHECK(ensure(h->small, pbytes) == CYX_OK, "partials allocation failed");
   hipStream_t st = h->ctx->stream;
   PhaseTimer tm(h->ctx, PH_GRAM);
+  // 16-byte vector loads when every lane's 32-byte run of a column is aligned
+  const auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const int vec = ((lda * sizeof(T)) % 16 == 0 && (ldb * sizeof(T)) % 16 == 0 && al(A) && al(B)) ? 1 : 0;
   hipLaunchKernelGGL(gram_mfma_kernel<T>, dim3(ntp, nchb), dim3(256), 0, st, (const T*)A, lda, na, (const T*)B, ldb,
-                     nb, W, N, chunk, herm ? 1 : 0, nta, (D*)h->small.p);
+                     nb, W, N, chunk, herm ? 1 : 0, nta, vec, (D*)h->small.p);
   CYX_HIP(hipGetLastError());
   hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / 256), dim3(256), 0, st, (const D*)h->small.p, nchb,
                      herm ? 1 : 0, nta, na, nb, (D*)M_out);

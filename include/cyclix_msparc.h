@@ -115,6 +115,10 @@ int cyx_ham_set_veff(cyx_ham_t ham, const double* veff, int on_device);
 int cyx_ham_destroy(cyx_ham_t ham);
 /* Device pointer of the FP64 weights W held by the handle (length N). */
 const double* cyx_ham_weights(cyx_ham_t ham);
+/* How many Rayleigh-Ritz rotations of FP32 blocks ran on the FP32 matrix
+ * cores (well-conditioned overlap) and how many on the FP64 ones, since the
+ * handle was made.  No counterpart in the reference. */
+int cyx_ham_rotation_counts(cyx_ham_t ham, int64_t* fp32, int64_t* fp64);
 
 /* Launch shape of the fused stencil kernel (the autotuner's knobs; no
  * counterpart in the reference).  tr: radial tile (0 = heuristic; clamped to

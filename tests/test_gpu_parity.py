@@ -163,6 +163,37 @@ def test_rayleigh_ritz_many_states(device, name, dt):
     ham.close()
 
 
+@pytest.mark.parametrize("name,dt", [("cyclic_gamma", "f32"), ("helical_bloch", "c64")])
+def test_fp32_rotation_gate(device, name, dt):
+    """FP32 blocks: the overlap runs on the FP32 matrix cores with FP64
+    chunk sums; when its Cholesky factor certifies the block well
+    conditioned, the projection and the rotation stay there, else all three
+    products run on the FP64 cores.  A random block takes the FP32 products;
+    a block with two nearly parallel columns the FP64 ones; both match the
+    oracle's Rayleigh-Ritz on the same block."""
+    from cyclix_msparc_amd.api import Hamiltonian
+    sysm, veff, proj, oham, dts = make_case(name)
+    ham = Hamiltonian(device, sysm, sysm.kpt, veff, proj)
+    N, ns = sysm.grid.N, 24
+    Wm = O.metric_weights(oham.g)
+    V = np.random.default_rng(8).standard_normal((N, 4))
+    for near, want in ((False, (1, 0)), (True, (1, 1))):
+        X = rand_block(N, ns, dt, 31)
+        if near:
+            X[:, 5] = X[:, 4] + 1e-3 * X[:, 5]
+        Xq = X.astype(NP[dt]).astype(X.dtype)
+        ev_ref, psi_ref = O.rayleigh_ritz_w(oham, Xq)
+        psi_d = to_dev(Xq, dt, device)
+        ev = ham.rayleigh_ritz(psi_d)
+        err = np.max(np.abs(ev - ev_ref)) / np.max(np.abs(ev_ref))
+        sub = rel(_subspace_projector(to_host(psi_d), Wm, V), _subspace_projector(psi_ref, Wm, V))
+        print(f"{name} {dt} near-parallel={near}: rotations (fp32, fp64) {ham.rotation_counts()}, "
+              f"eigenvalue rel err {err:.2e}, subspace err {sub:.2e}", flush=True)
+        assert ham.rotation_counts() == want
+        assert err < 2e-5 and sub < 1e-3
+    ham.close()
+
+
 @pytest.mark.parametrize("name,dt,ns", [("cyclic_gamma", "f64", 150), ("cyclic_gamma", "f64", 256),
                                         ("helical_bloch", "c128", 150), ("helical_bloch", "c128", 256)])
 def test_rayleigh_ritz_tridiagonal_paths(device, name, dt, ns, monkeypatch):
