@@ -88,6 +88,54 @@ class GridSpec:
         return np.kron(np.full(self.nz, self.dz), np.kron(np.full(self.nt, self.dt), wx))
 
 
+def _matlab_round(x):
+    """MATLAB round: halves away from zero (numpy rounds halves to even)."""
+    x = np.asarray(x, dtype=float)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def generate_kpts(nkpt, L1, L2, L3, kptshift=(0.0, 0.0, 0.0), cell_typ=3, time_rev=True):
+    """Bloch channels of a cyclic / helical sector and their weights:
+    ``Generate_kpts`` (``src/initialization.m:1591-1682``, the cell_typ 3-5
+    branch).  The cyclic channel is the integer nu = round(2 pi/L2 * j/n2),
+    j = 0..n2-1 (MATLAB rounding); the axial one eta = 2 pi/L3 * k with k on
+    the Monkhorst-Pack grid of n3 points (shifted, mapped into (-1/2, 1/2]).
+    Time-reversal symmetry drops channel i when a later channel j has
+    nu_i + nu_j in {0, 2 pi/L2} and eta_i + eta_j = 0, and doubles j's
+    weight.  Returns (kptgrid (n, 3) rows [0, nu, eta], weights (n,))."""
+    n1, n2, n3 = (int(v) for v in nkpt)
+    if n1 != 1:
+        raise ValueError("nkpt cannot be greater than 1 in the Dirichlet radial direction")
+    eps = np.finfo(float).eps
+    mp1 = lambda n: np.arange(-((n - 1) // 2), -((n - 1) // 2) + n)   # noqa: E731  (MP grid, infinite order)
+    kx = mp1(n1) / n1
+    ky = np.arange(n2) / n2                                              # finite group order
+    kz = mp1(n3) / n3 + kptshift[2] / n3
+    kz = np.mod(kz + 0.5 - eps, 1.0) - 0.5 + eps
+    kx = (2 * math.pi / L1) * kx
+    ky = _matlab_round((2 * math.pi / L2) * ky)
+    kz = (2 * math.pi / L3) * kz
+    # ndgrid order: x fastest, then y, then z
+    grid = np.array([[x, y, z] for z in kz for y in ky for x in kx], dtype=float)
+    n = grid.shape[0]
+    w = np.full(n, 1.0 / n)
+    if time_rev:
+        sumy = 2 * math.pi / L2
+        tol = 1e-8
+        drop = np.zeros(n, dtype=bool)
+        dbl = np.zeros(n, dtype=bool)
+        for i in range(n):
+            for j in range(i + 1, n):
+                if (abs(grid[i, 0] + grid[j, 0]) < tol
+                        and (abs(grid[i, 1] + grid[j, 1] - sumy) < tol or abs(grid[i, 1] + grid[j, 1]) < tol)
+                        and abs(grid[i, 2] + grid[j, 2]) < tol):
+                    drop[i] = True
+                    dbl[j] = True
+        w[dbl] *= 2
+        grid, w = grid[~drop], w[~drop]
+    return grid, w
+
+
 def to_cart(g: GridSpec, pts):
     """(r, theta, z) -> Cartesian.  ``Cychel/coordinateTransformation_cychel.m:3-7``."""
     pts = np.atleast_2d(pts)

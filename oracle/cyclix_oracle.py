@@ -103,6 +103,42 @@ def intg_weights(g: Grid) -> np.ndarray:
     return np.kron(wz, np.kron(wy, wx))
 
 
+def bloch_channels(nkpt, L1, L2, L3, kptshift=(0.0, 0.0, 0.0), time_rev=True):
+    """Cyclix branch of ``Generate_kpts`` (``src/initialization.m:1591-1682``):
+    channels [0, nu, eta] with nu = round(N_c j / n2) (N_c = 2 pi / L2, MATLAB
+    round-half-away), eta = 2 pi/L3 * (Monkhorst-Pack point + shift / n3) in
+    (-1/2, 1/2], weights 1/n; under time reversal a channel whose partner
+    (nu' = N_c - nu or -nu, eta' = -eta) appears later in the ndgrid order is
+    dropped and the partner's weight doubled.  Returns (list of 3-tuples,
+    weights)."""
+    n1, n2, n3 = nkpt
+    assert n1 == 1
+    Nc = 2 * math.pi / L2
+    nus = [math.floor(Nc * j / n2 + 0.5) for j in range(n2)]
+    start = -((n3 - 1) // 2)
+    etas = []
+    for k in range(n3):
+        f = (start + k) / n3 + kptshift[2] / n3
+        f = ((f + 0.5 - np.finfo(float).eps) % 1.0) - 0.5 + np.finfo(float).eps
+        etas.append(2 * math.pi / L3 * f)
+    chans = [(0.0, float(nu), eta) for eta in etas for nu in nus]
+    w = [1.0 / len(chans)] * len(chans)
+    if not time_rev:
+        return chans, np.array(w)
+    keep, wk = [], []
+    partnered = set()
+    for i, (_, nu_i, eta_i) in enumerate(chans):
+        later = [j for j in range(i + 1, len(chans))
+                 if (abs(nu_i + chans[j][1] - Nc) < 1e-8 or abs(nu_i + chans[j][1]) < 1e-8)
+                 and abs(eta_i + chans[j][2]) < 1e-8]
+        partnered.update(later)
+        if not later:
+            keep.append(i)
+    out = [chans[i] for i in keep]
+    weights = np.array([w[i] * (2.0 if i in partnered else 1.0) for i in keep])
+    return out, weights
+
+
 def metric_weights(g: Grid) -> np.ndarray:
     """W' = r dr dtheta dz without the end corrections of ``intg_weights``.
     The identity k w2_k = 2 w1_k (``src/initialization.m:407-417``) makes

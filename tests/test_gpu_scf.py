@@ -90,3 +90,39 @@ def test_mixed_precision_scf_error_against_fp64_oracle(device):
     assert np.max(np.abs(res.EigVal[:nocc] - ref["EigVal"][:nocc])) <= 1e-6
     print(f"mixed-precision error: dE/atom = {res.Eatom - ref['Eatom']:.3e} Ha, "
           f"max |d eig| = {np.max(np.abs(res.EigVal[:nocc] - ref['EigVal'][:nocc])):.3e} Ha")
+
+
+def test_multichannel_scf_matches_oracle(device):
+    """Two Bloch channels from Generate_kpts (src/initialization.m:1591-1682)
+    on the 1/6 sector with KPOINT_GRID 1 3 1: nu = 0 (real, weight 1/3) and
+    nu = 4 (complex, weight 2/3, standing for nu = 2 by time reversal).  The
+    device SCF (density summed over channels with their weights, one Fermi
+    level over both) reaches the oracle's fixed point."""
+    from cyclix_msparc_amd.scf import GroundState, SCFOptions
+    from cyclix_msparc_amd.system import generate_kpts, scf_system
+    from tests.scf_common import oracle_inputs
+    sysm = scf_system()
+    g0 = sysm.grid
+    L1 = (g0.nr - 1) * g0.dr
+    kpts, w = generate_kpts((1, 3, 1), L1, g0.L2, g0.L3)
+    assert kpts[:, 1].tolist() == [0.0, 4.0] and np.allclose(w, [1 / 3, 2 / 3])
+    rng = np.random.default_rng(2)
+    N, ns = g0.N, sysm.nstates
+    psi0 = [rng.standard_normal((N, ns)), rng.standard_normal((N, ns)) + 1j * rng.standard_normal((N, ns))]
+    g, atoms, pc = oracle_inputs(sysm)
+    ref = O.scf(g, sysm.W, atoms, [tuple(k) for k in kpts], list(w), pc["b"], pc["rho_at"], pc["Eself"],
+                pc["E_corr"], sysm.nelectron, sysm.n_atm, psi0, sysm.npl, 1e-7, 1e-9)
+    gs = GroundState(device, sysm, channels=[(tuple(k), wk) for k, wk in zip(kpts, w)],
+                     options=SCFOptions(SCF_tol=1e-7, poisson_tol=1e-9))
+    res = gs.run(psi0=[p.T for p in psi0])
+    assert res.converged and ref["converged"]
+    assert gs.psi[0].dtype == torch.float64 and gs.psi[1].dtype == torch.complex128
+    dE = abs(res.Eatom - ref["Eatom"])
+    occ = res.occ > 1e-6
+    dl = float(np.max(np.abs(res.EigVal - ref["EigVal"])[occ]))
+    print(f"two-channel SCF: {res.count_SCF} iterations (oracle {ref['count_SCF']}), |dE/atom| {dE:.2e}, "
+          f"max occupied |d lambda| {dl:.2e}, lambda_f {res.lambda_f:.8f} vs {ref['lambda_f']:.8f}")
+    assert dE <= 1e-7
+    assert dl <= 1e-6
+    assert res.lambda_f == pytest.approx(ref["lambda_f"], abs=1e-6)
+    gs.close()
