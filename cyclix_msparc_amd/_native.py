@@ -26,11 +26,13 @@ EXPORTS = (
     "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
-    "cyx_sternheimer_apply", "cyx_sternheimer_solve",
+    "cyx_sternheimer_apply", "cyx_sternheimer_solve", "cyx_sternheimer_plan_create", "cyx_sternheimer_plan_iterate",
+    "cyx_sternheimer_plan_status", "cyx_sternheimer_plan_destroy", "cyx_sternheimer_rhs",
     "cyx_unet_create", "cyx_unet_destroy", "cyx_unet_num_params", "cyx_unet_set_params", "cyx_unet_get_params",
     "cyx_unet_forward", "cyx_unet_gradient", "cyx_unet_train_step", "cyx_unet_input",
     "cyx_poisson_set_preconditioner", "cyx_poisson_apply",
     "cyx_filter_plan_create", "cyx_filter_plan_run", "cyx_filter_plan_tuning", "cyx_filter_plan_destroy",
+    "cyx_filter_plan_attach_sternheimer",
 )
 
 
@@ -136,6 +138,14 @@ def load():
         "cyx_sternheimer_apply": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32]),
         "cyx_sternheimer_solve": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32, i32, dbl, i32, i32,
                                         C.POINTER(SternInfo), C.POINTER(C.c_int32), pd]),
+        "cyx_sternheimer_plan_create": (i32, [vp, i32, vp, vp, i32, i64, pd, vp, i64, i32, i32, dbl, i32,
+                                              C.POINTER(vp)]),
+        "cyx_sternheimer_plan_iterate": (i32, [vp, i32]),
+        "cyx_sternheimer_plan_status": (i32, [vp, C.POINTER(C.c_int32), C.POINTER(SternInfo),
+                                              C.POINTER(C.c_int32), pd]),
+        "cyx_sternheimer_plan_destroy": (i32, [vp]),
+        "cyx_sternheimer_rhs": (i32, [vp, i32, vp, i64, i32, pd, vp, i64, pd]),
+        "cyx_filter_plan_attach_sternheimer": (i32, [vp, vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

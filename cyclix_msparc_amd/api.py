@@ -338,11 +338,105 @@ class Hamiltonian:
                                                  nat.dptr(rel)), allow=allow)
         return X, info, its, rel
 
+    def sternheimer_rhs(self, psi, nv, coeff, out=None):
+        """Right-hand sides of one perturbation (``src/phonon.m:1358-1444``) on
+        this handle as the first-order operator: its potential is dVeff and its
+        projectors are the pairs (chi +- dchi)/sqrt(2) with gammas +-gamma
+        (``dfpt.perturbation_projectors``).  psi: (>= nv, N) block of the
+        channel's states; coeff: nv x nv (``dfpt.coeff_pv_psi``).  Returns the
+        (nv, N) block B and dlambda (S.Dlambda_dR) on the host."""
+        _check_block(psi, self.N)
+        nv = int(nv)
+        if not 1 <= nv <= psi.shape[0]:
+            raise ValueError("bad number of occupied states")
+        cf = np.asfortranarray(np.asarray(coeff, dtype=float))
+        if cf.shape != (nv, nv):
+            raise ValueError("coeff must be nv x nv")
+        B = torch.empty((nv, self.N), dtype=psi.dtype, device=psi.device) if out is None else out
+        _check_block(B, self.N)
+        if B.dtype != psi.dtype or B.shape[0] < nv:
+            raise ValueError("out must share psi's dtype and hold nv columns")
+        dl = np.zeros(nv)
+        cfc = np.ascontiguousarray(cf.ravel(order="F"))
+        nat.check(self.lib.cyx_sternheimer_rhs(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()), self.N, nv,
+                                               nat.dptr(cfc), C.c_void_p(B.data_ptr()), self.N, nat.dptr(dl)))
+        return B, dl
+
     def lanczos_bounds(self, v0, maxit=300, tol=1e-3):
         v0 = np.ascontiguousarray(np.asarray(v0, dtype=float))
         lo, hi = C.c_double(), C.c_double()
         nat.check(self.lib.cyx_lanczos_bounds(self.h, nat.dptr(v0), int(maxit), float(tol), C.byref(lo), C.byref(hi)))
         return lo.value, hi.value
+
+
+class SternheimerPlan:
+    """Batched BiCGSTAB on the Sternheimer systems with its iterations
+    captured in a HIP graph (cyx_sternheimer_plan_*): converged columns
+    freeze on the device, the host only reads the active count between
+    replays.  B: (ncol, N) right-hand sides; lam: one shift per column; psi_v:
+    (nv, N) occupied states or None.  The solution is ``self.X``."""
+
+    def __init__(self, ham: Hamiltonian, B, lam, psi_v=None, X0=None, tol=1e-8, precond=True):
+        self.dev = ham.dev
+        self.lib = ham.lib
+        self.ham = ham
+        _check_block(B, ham.N)
+        lam = np.ascontiguousarray(np.asarray(lam, dtype=float))
+        if lam.size != B.shape[0]:
+            raise ValueError("one shift per column")
+        nv = 0 if psi_v is None else psi_v.shape[0]
+        if psi_v is not None:
+            _check_block(psi_v, ham.N)
+            if psi_v.dtype != B.dtype:
+                raise ValueError("psi_v and B must share a dtype")
+        # the captured graph holds these addresses: keep the tensors alive
+        self.B, self.psi_v, self.lam = B, psi_v, lam
+        self.X = X0.clone().contiguous() if X0 is not None else torch.empty_like(B)
+        self.ncol = B.shape[0]
+        self.tol = float(tol)
+        h = C.c_void_p()
+        nat.check(self.lib.cyx_sternheimer_plan_create(ham.h, _dtype_code(B), C.c_void_p(B.data_ptr()),
+                                                       C.c_void_p(self.X.data_ptr()), self.ncol, ham.N,
+                                                       nat.dptr(lam), C.c_void_p(psi_v.data_ptr()) if nv else None,
+                                                       ham.N, nv, 1 if X0 is not None else 0, self.tol,
+                                                       1 if precond else 0, C.byref(h)))
+        self.h = h
+        self.dev.register(self)
+
+    def iterate(self, iters=4):
+        nat.check(self.lib.cyx_sternheimer_plan_iterate(self.h, int(iters)))
+
+    def status(self):
+        """(active columns, SternInfo, per-column iterations, relative residuals)."""
+        act = C.c_int32()
+        info = nat.SternInfo()
+        its = np.zeros(self.ncol, dtype=np.int32)
+        rel = np.zeros(self.ncol)
+        nat.check(self.lib.cyx_sternheimer_plan_status(self.h, C.byref(act), C.byref(info), nat.iptr(its),
+                                                       nat.dptr(rel)))
+        return act.value, info, its, rel
+
+    def solve(self, max_iter=3000, poll=8):
+        """Replay until every column meets tol or max_iter iterations ran."""
+        done = 0
+        act = self.status()[0]
+        while act > 0 and done < max_iter:
+            k = min(poll, max_iter - done)
+            self.iterate(k)
+            done += k
+            act = self.status()[0]
+        return self.status()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cyx_sternheimer_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FilterPlan:
@@ -388,6 +482,13 @@ class FilterPlan:
         t = nat.Tuning()
         nat.check(self.lib.cyx_filter_plan_tuning(self.h, C.byref(t)))
         return {"tr": t.tr, "flags": t.flags}
+
+    def attach_sternheimer(self, sp, iters=4):
+        """Run ``iters`` iterations of the Sternheimer plan ``sp`` in every
+        pass's graph launch, as an independent branch (None detaches)."""
+        nat.check(self.lib.cyx_filter_plan_attach_sternheimer(self.h, sp.h if sp is not None else None,
+                                                              int(iters)))
+        self.stern = sp
 
     def run(self, a, b, a0):
         """Filter every block in place with per-channel bounds."""

@@ -756,8 +756,8 @@ int cyx_ctx_set_stream(cyx_ctx_t c, void* s) {
   return CYX_OK;
 }
 
-static const char* kPhaseNames[PH_COUNT] = {"stencil", "h_apply", "kb_gather", "gram", "rotate",
-                                            "wnorm",   "dense_eig", "aar",     "rr_apply"};
+static const char* kPhaseNames[PH_COUNT] = {"stencil", "h_apply",  "kb_gather", "gram",       "rotate",
+                                            "wnorm",   "dense_eig", "aar",      "rr_apply", "sternheimer"};
 
 static int resolve_marks(cyx_ctx_s* c) {
   if (c->marks.empty()) return CYX_OK;

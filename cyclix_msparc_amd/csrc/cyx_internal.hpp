@@ -94,6 +94,7 @@ enum Phase {
   PH_DENSE_EIG,
   PH_AAR,
   PH_RR_APPLY,
+  PH_STERN,
   PH_COUNT
 };
 }
@@ -205,13 +206,20 @@ int stencil_step(cyx_ham_s* h, int dtype, const void* in, const void* prev, void
 // storage type of in/out (FP32 blocks are widened on load and rounded on
 // store): the Rayleigh-Ritz and residual H-applies of the mixed path
 int stencil_apply_fp64(cyx_ham_s* h, int dtype, const void* in, void* out, int ncol, int64_t ld);
+// Y = alpha (V X + KB X): the handle's potential and projector terms without
+// the kinetic stencil (X: ldx, Y: ldy, Y != X)
+int potential_apply(cyx_ham_s* h, int dtype, const void* X, int64_t ldx, void* Y, int64_t ldy, int ncol,
+                    double alpha, const KbScratch* ks = nullptr);
 
 // subspace kernels (FP64 accumulation)
 int gram_w(cyx_ham_s* h, int dtype, const void* A, const void* B, int ns, int64_t ld, void* M_out /*dev*/);
 // M (na x nb, FP64 / complex FP64, column-major) = A^H diag(W) B over the N grid points;
 // herm: the product is Hermitian (na == nb), only the upper tiles are computed and mirrored
+// part: device scratch of gram_part_bytes bytes for the partial sums (null:
+// the handle's lazily grown buffer; captured graphs pass their own)
 int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-              const double* W, void* M_out /*dev*/, bool herm = false);
+              const double* W, void* M_out /*dev*/, bool herm = false, void* part = nullptr);
+size_t gram_part_bytes(int dtype, int64_t N, int na, int nb, bool herm);
 int rotate(cyx_ham_s* h, int dtype, const void* A, const void* Q /*dev fp64 ns x ns*/, void* out, int ns, int64_t ld);
 // the same products of FP32 blocks on the FP32 matrix cores (cyx_subspace32.hip):
 // the Gram with FP32 sums over 128-point chunks added in FP64; the rotation
@@ -262,5 +270,11 @@ int energy_terms(hipStream_t st, int64_t N, const double* rho, const double* vxc
 int veff_from_device(cyx_ham_s* h, const double* veff);
 
 size_t dtype_size(int dtype);
+
+// Sternheimer plans (cyx_sternheimer.hip): enqueue `iters` BiCGSTAB
+// iterations on st (the context's stream is switched to st and timing is off
+// meanwhile, so that the launches can be captured into a caller's graph)
+int stern_plan_enqueue(cyx_stern_plan_s* sp, hipStream_t st, int iters);
+cyx_ctx_s* stern_plan_context(const cyx_stern_plan_s* sp);
 
 }  // namespace cyx

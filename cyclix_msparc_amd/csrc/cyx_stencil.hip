@@ -508,6 +508,61 @@ int stencil_apply_fp64(cyx_ham_s* h, int dtype, const void* in, void* out, int n
   return CYX_ERR_ARG;
 }
 
+// ---------------------------------------------------------------- potential-only operator
+// Y = alpha V X (FP64 arithmetic), the local part of potential_apply
+template <class T>
+__global__ void __launch_bounds__(256) pot_mul_kernel(const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
+                                                      int64_t ldy, const double* __restrict__ V, int64_t N,
+                                                      double alpha) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const int s = blockIdx.y;
+  Y[(int64_t)s * ldy + n] = narrow<T>(rmul(alpha * V[n], widen(X[(int64_t)s * ldx + n])));
+}
+
+template <class T>
+static int potential_impl(cyx_ham_s* h, const void* X, int64_t ldx, void* Y, int64_t ldy, int ncol, double alpha,
+                          const KbScratch* ks) {
+  hipStream_t st = h->ctx->stream;
+  const int64_t N = h->N();
+  const KbTables<T>& K = kb_of<T>(h);
+  CYX_CHECK((size_t)KSC * K.nproj * sizeof(T) <= 32 * 1024, "too many projectors for the KB scatter (%d)", K.nproj);
+  T* C = nullptr;
+  int rc;
+  {
+    PhaseTimer tm(h->ctx, PH_KB_GATHER);
+    if ((rc = gather_launch<T>(h, X, ncol, ldx, st, ks, &C)) != CYX_OK) return rc;
+  }
+  PhaseTimer tm(h->ctx, PH_HAPPLY);
+  hipLaunchKernelGGL(pot_mul_kernel<T>, dim3(ceil_div(N, 256), ncol), dim3(256), 0, st, (const T*)X, ldx, (T*)Y, ldy,
+                     (const double*)h->c64.veff, N, alpha);
+  CYX_HIP(hipGetLastError());
+  if (K.nproj > 0 && K.nkbn > 0) {
+    hipLaunchKernelGGL((kb_scatter_kernel<T, T>), dim3(ceil_div(K.nkbn, 256), ceil_div(ncol, KSC)), dim3(256),
+                       (size_t)KSC * K.nproj * sizeof(T), st, (T*)Y, ldy, ncol, (const int*)K.kb_nodes, K.nkbn,
+                       (const uint32_t*)K.s_pk, (const T*)K.s_cph, (const int*)K.s_pj, (const T*)C, K.nproj,
+                       (typename traits<T>::real)alpha);
+    CYX_HIP(hipGetLastError());
+  }
+  return CYX_OK;
+}
+
+int potential_apply(cyx_ham_s* h, int dtype, const void* X, int64_t ldx, void* Y, int64_t ldy, int ncol,
+                    double alpha, const KbScratch* ks) {
+  switch (dtype) {
+    case CYX_F64:
+      return potential_impl<double>(h, X, ldx, Y, ldy, ncol, alpha, ks);
+    case CYX_F32:
+      return potential_impl<float>(h, X, ldx, Y, ldy, ncol, alpha, ks);
+    case CYX_C128:
+      return potential_impl<cplx<double>>(h, X, ldx, Y, ldy, ncol, alpha, ks);
+    case CYX_C64:
+      return potential_impl<cplx<float>>(h, X, ldx, Y, ldy, ncol, alpha, ks);
+  }
+  set_error("unknown dtype %d", dtype);
+  return CYX_ERR_ARG;
+}
+
 }  // namespace cyx
 
 // ---------------------------------------------------------------- batched filter plans
@@ -536,6 +591,12 @@ struct cyx_filter_plan_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;  // capture stream (the context's stream may be the null stream)
+  // attached Sternheimer iterations: a second branch of the graph, captured
+  // on cap2 between a fork and a join event
+  cyx_stern_plan_s* stern = nullptr;
+  int stern_iters = 0;
+  hipStream_t cap2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -659,10 +720,30 @@ int plan_run(cyx_filter_plan_s* pl, const double* a, const double* b, const doub
     // capture the pass once, on a stream of the plan's own (capture is not
     // allowed on the null stream); later passes replay it with new scalars
     if (!pl->cap) CYX_HIP(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+    if (pl->stern && !pl->cap2) {
+      CYX_HIP(hipStreamCreateWithFlags(&pl->cap2, hipStreamNonBlocking));
+      CYX_HIP(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
+      CYX_HIP(hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming));
+    }
     CYX_HIP(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+    int rc2 = CYX_OK;
+    hipError_t e2 = hipSuccess;
+    if (pl->stern) {
+      // the Sternheimer branch forks from the graph's start and joins at its
+      // end; it shares no buffer with the filter steps
+      e2 = hipEventRecord(pl->ev_fork, pl->cap);
+      if (e2 == hipSuccess) e2 = hipStreamWaitEvent(pl->cap2, pl->ev_fork, 0);
+      if (e2 == hipSuccess) rc2 = stern_plan_enqueue(pl->stern, pl->cap2, pl->stern_iters);
+      if (e2 == hipSuccess) e2 = hipEventRecord(pl->ev_join, pl->cap2);
+    }
     plan_enqueue<T>(pl, pl->cap, pl->m);
+    if (pl->stern && e2 == hipSuccess) e2 = hipStreamWaitEvent(pl->cap, pl->ev_join, 0);
     hipError_t e = hipStreamEndCapture(pl->cap, &pl->graph);
-    if (e != hipSuccess) {
+    if (e == hipSuccess) e = e2;
+    if (rc2 != CYX_OK || e != hipSuccess) {
+      if (pl->graph) (void)hipGraphDestroy(pl->graph);
+      pl->graph = nullptr;
+      if (rc2 != CYX_OK) return rc2;
       set_error("graph capture failed: %s", hipGetErrorString(e));
       return CYX_ERR_HIP;
     }
@@ -817,12 +898,31 @@ int cyx_filter_plan_tuning(cyx_filter_plan_t pl, cyx_tuning* t) {
   return CYX_OK;
 }
 
+int cyx_filter_plan_attach_sternheimer(cyx_filter_plan_t pl, cyx_stern_plan_t sp, int iters) {
+  CYX_CHECK(pl && (!sp || iters >= 1), "bad argument");
+  CYX_CHECK(!sp || stern_plan_context(sp) == pl->ctx, "the Sternheimer plan belongs to another context");
+  // the next run captures the graph again
+  if (pl->exec) {
+    CYX_HIP(hipStreamSynchronize(pl->ctx->stream));
+    (void)hipGraphExecDestroy(pl->exec);
+  }
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  pl->exec = nullptr;
+  pl->graph = nullptr;
+  pl->stern = sp;
+  pl->stern_iters = sp ? iters : 0;
+  return CYX_OK;
+}
+
 int cyx_filter_plan_destroy(cyx_filter_plan_t pl) {
   if (!pl) return CYX_OK;
   (void)hipDeviceSynchronize();  // the context may already be gone
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
+  if (pl->cap2) (void)hipStreamDestroy(pl->cap2);
+  if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
+  if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
   (void)hipFree(pl->d_steps);
   (void)hipFree(pl->d_gather);
   for (void* p : pl->kbC) (void)hipFree(p);

@@ -39,8 +39,9 @@ struct ColState {
 
 constexpr int SJB = 8;  // columns per thread in the finishing kernel
 
-// HX[j] = HX[j] - lam[j] X[j] + sum_v PV[v] C[v, j]   (C column-major nv x ncol)
-template <class T>
+// HX[j] = HX[j] - lam[j] X[j] + sum_v PV[v] C[v, j]   (C column-major nv x ncol);
+// SHIFT = false: HX[j] += sum_v PV[v] C[v, j] (X and lam unused)
+template <class T, bool SHIFT = true>
 __global__ void __launch_bounds__(256) stern_finish_kernel(T* __restrict__ HX, const T* __restrict__ X, int64_t ld,
                                                            const T* __restrict__ PV, int64_t ldv, int nv,
                                                            const typename acc_of<T>::type* __restrict__ C,
@@ -64,8 +65,12 @@ __global__ void __launch_bounds__(256) stern_finish_kernel(T* __restrict__ HX, c
     if (jj < jn) {
       const int64_t o = (int64_t)(j0 + jj) * ld + n;
       const D hx = widen(HX[o]);
-      const D x = widen(X[o]);
-      HX[o] = narrow<T>(add(rfma(-lam[j0 + jj], x, hx), acc[jj]));
+      if constexpr (SHIFT) {
+        const D x = widen(X[o]);
+        HX[o] = narrow<T>(add(rfma(-lam[j0 + jj], x, hx), acc[jj]));
+      } else {
+        HX[o] = narrow<T>(add(hx, acc[jj]));
+      }
     }
   }
 }
@@ -348,57 +353,80 @@ __global__ void dinv_kernel(const double* rcoef, int nr, double* dinv) {
   if (i < nr) dinv[i] = 1.0 / rcoef[(int64_t)i * RCOEF + RCOEF - 1];
 }
 
+// F = coeff .* (-G) and dlam_n = -Re G_nn, where G = psi^H W B with B = -dH psi
+// (so -G is psi_DH_psi of src/phonon.m:1360,1402-1405)
+template <class D>
+__global__ void __launch_bounds__(256) rhs_coef_kernel(const D* __restrict__ G, const double* __restrict__ coeff,
+                                                       D* __restrict__ F, double* __restrict__ dlam, int nv) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= nv * nv) return;
+  const D dh = neg(G[idx]);
+  F[idx] = dscale(dh, coeff[idx]);
+  if (idx % nv == idx / nv) dlam[idx / nv] = re(dh);
+}
+
 // ---------------------------------------------------------------- host side
 struct SternLayout {
-  size_t blk;  // bytes of one ncol x N block
-  char* base;
-  void* r;
-  void* rt;
-  void* p;
-  void* v;
-  void* ph;
-  void* sh;
-  void* t;
-  void* C;
-  double* lam;
-  void* part;
-  void* state;
-  double* dinv;
-  int* count;
+  size_t blk = 0;  // bytes of one ncol x N block
+  void* r = nullptr;
+  void* rt = nullptr;
+  void* p = nullptr;
+  void* v = nullptr;
+  void* ph = nullptr;
+  void* sh = nullptr;
+  void* t = nullptr;
+  void* C = nullptr;
+  double* lam = nullptr;
+  void* part = nullptr;
+  void* state = nullptr;
+  double* dinv = nullptr;
+  int* count = nullptr;
+  // KB coefficients / partials of the stencil steps and the Gram partials:
+  // in the workspace when `own` (a captured graph must not use the handle's
+  // lazily grown buffers), else null (the handle's)
+  bool own = false;
+  KbScratch ks{nullptr, nullptr};
+  void* gpart = nullptr;
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Workspace of ncol columns against nv occupied states: the seven BiCGSTAB
+// blocks (solver only), the projections C, the shifts, the column-dot
+// partials, the column states, the Jacobi diagonal, the active count and,
+// when own, the step scratch.  Returns the bytes; assigns the pointers when
+// base is not null.
 template <class T>
-static int stern_layout(cyx_ham_s* h, int ncol, int nv, int nchunk, bool solver, SternLayout& L) {
+static size_t stern_layout(cyx_ham_s* h, int dtype, int ncol, int nv, int nchunk, bool solver, bool own, char* base,
+                           SternLayout& L) {
   using D = typename acc_of<T>::type;
   const int64_t N = h->N();
+  size_t off = 0;
+  const auto take = [&](size_t bytes) -> char* {
+    char* q = base ? base + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
   L.blk = align256((size_t)ncol * N * sizeof(T));
-  const size_t nblk = solver ? 7 : 0;
-  const size_t bC = align256((size_t)nv * ncol * sizeof(D) + 16);
-  const size_t bl = align256((size_t)ncol * sizeof(double));
-  const size_t bp = align256((size_t)ncol * nchunk * 2 * sizeof(D));
-  const size_t bs = align256((size_t)ncol * sizeof(ColState<D>));
-  const size_t bd = align256((size_t)h->g.nr * sizeof(double));
-  const size_t total = nblk * L.blk + bC + bl + bp + bs + bd + 256;
-  if (ensure(h->stern, total) != CYX_OK) return CYX_ERR_NOMEM;
-  char* b = (char*)h->stern.p;
-  L.base = b;
   void** blocks[7] = {&L.r, &L.rt, &L.p, &L.v, &L.ph, &L.sh, &L.t};
-  for (size_t k = 0; k < 7; ++k) *blocks[k] = k < nblk ? (void*)(b + k * L.blk) : nullptr;
-  b += nblk * L.blk;
-  L.C = b;
-  b += bC;
-  L.lam = (double*)b;
-  b += bl;
-  L.part = b;
-  b += bp;
-  L.state = b;
-  b += bs;
-  L.dinv = (double*)b;
-  b += bd;
-  L.count = (int*)b;
-  return CYX_OK;
+  for (void** b : blocks) *b = solver ? take(L.blk) : nullptr;
+  L.C = take((size_t)nv * ncol * sizeof(D) + 16);
+  L.lam = (double*)take((size_t)ncol * sizeof(double));
+  L.part = take((size_t)ncol * nchunk * 2 * sizeof(D));
+  L.state = take((size_t)ncol * sizeof(ColState<D>));
+  L.dinv = (double*)take((size_t)h->g.nr * sizeof(double));
+  L.count = (int*)take(sizeof(int));
+  L.own = own;
+  L.ks = KbScratch{nullptr, nullptr};
+  L.gpart = nullptr;
+  if (own) {
+    // kb_scratch_bytes' layout: C rounded up to 256 bytes, then the partials
+    const size_t kb = kb_scratch_bytes(h, dtype, ncol);
+    char* k = take(std::max<size_t>(kb, 1));
+    if (k) L.ks = KbScratch{k, k + align256((size_t)ncol * h->nproj * sizeof(T))};
+    if (nv > 0) L.gpart = take(gram_part_bytes(dtype, N, nv, ncol, false));
+  }
+  return off + 256;
 }
 
 // Y = A X for all columns (Y != X)
@@ -413,9 +441,10 @@ static int stern_op(cyx_ham_s* h, int dtype, const T* X, int64_t ldx, T* Y, int6
     set_error("internal: Sternheimer operator needs equal leading dimensions");
     return CYX_ERR_ARG;
   }
-  if ((rc = stencil_step(h, dtype, X, nullptr, Y, ncol, ldy, 1.0, 0.0, 0.0)) != CYX_OK) return rc;
+  if ((rc = stencil_step(h, dtype, X, nullptr, Y, ncol, ldy, 1.0, 0.0, 0.0, L.own ? &L.ks : nullptr)) != CYX_OK)
+    return rc;
   if (nv > 0) {
-    if ((rc = gram_rect(h, dtype, PV, ldv, nv, X, ldx, ncol, h->W, L.C)) != CYX_OK) return rc;
+    if ((rc = gram_rect(h, dtype, PV, ldv, nv, X, ldx, ncol, h->W, L.C, false, L.gpart)) != CYX_OK) return rc;
   }
   hipLaunchKernelGGL(stern_finish_kernel<T>, dim3(ceil_div(N, 256), ceil_div(ncol, SJB)), dim3(256), 0,
                      h->ctx->stream, Y, X, ldy, PV, ldv, nv, (const D*)L.C, (const double*)L.lam, N, ncol);
@@ -427,97 +456,353 @@ template <class T>
 static int stern_apply_impl(cyx_ham_s* h, int dtype, const void* X, void* Y, int ncol, int64_t ld,
                             const double* lam, const void* PV, int64_t ldv, int nv) {
   SternLayout L;
-  CYX_CHECK(stern_layout<T>(h, ncol, nv, 1, false, L) == CYX_OK, "Sternheimer workspace allocation failed");
+  const size_t bytes = stern_layout<T>(h, dtype, ncol, nv, 1, false, false, nullptr, L);
+  CYX_CHECK(ensure(h->stern, bytes) == CYX_OK, "Sternheimer workspace allocation failed");
+  stern_layout<T>(h, dtype, ncol, nv, 1, false, false, (char*)h->stern.p, L);
   CYX_HIP(hipMemcpyAsync(L.lam, lam, ncol * sizeof(double), hipMemcpyHostToDevice, h->ctx->stream));
   return stern_op<T>(h, dtype, (const T*)X, ld, (T*)Y, ld, ncol, (const T*)PV, ldv, nv, L);
 }
 
+// B = coeff-weighted occupied part - dH psi (src/phonon.m:1440-1444), see cyx_sternheimer_rhs
 template <class T>
-static int stern_solve_impl(cyx_ham_s* h, int dtype, const void* B, void* X, int ncol, int64_t ld, const double* lam,
-                            const void* PV, int64_t ldv, int nv, int use_guess, double tol, int maxit,
-                            int precond, cyx_stern_info* info, int32_t* col_iters, double* col_relres) {
+static int stern_rhs_impl(cyx_ham_s* h, int dtype, const void* psi, int64_t ldpsi, int nv, const double* coeff,
+                          void* B, int64_t ldb, double* dlambda) {
   using D = typename acc_of<T>::type;
   const int64_t N = h->N();
   hipStream_t st = h->ctx->stream;
-  const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(128, ceil_div(N, 2048)));
-  SternLayout L;
-  CYX_CHECK(stern_layout<T>(h, ncol, nv, nchunk, true, L) == CYX_OK, "Sternheimer workspace allocation failed");
+  const size_t bG = align256((size_t)nv * nv * sizeof(D));
+  const size_t bc = align256((size_t)nv * nv * sizeof(double));
+  const size_t bl = align256((size_t)nv * sizeof(double));
+  CYX_CHECK(ensure(h->stern, 2 * bG + bc + bl) == CYX_OK, "Sternheimer workspace allocation failed");
+  char* b = (char*)h->stern.p;
+  D* G = (D*)b;
+  D* F = (D*)(b + bG);
+  double* cf = (double*)(b + 2 * bG);
+  double* dl = (double*)(b + 2 * bG + bc);
+  int rc;
+  // B = -dH psi
+  if ((rc = potential_apply(h, dtype, psi, ldpsi, B, ldb, nv, -1.0)) != CYX_OK) return rc;
+  // G = psi^H W B = -psi_DH_psi
+  if ((rc = gram_rect(h, dtype, psi, ldpsi, nv, B, ldb, nv, h->W, G, false)) != CYX_OK) return rc;
+  CYX_HIP(hipMemcpyAsync(cf, coeff, (size_t)nv * nv * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(rhs_coef_kernel<D>, dim3(ceil_div(nv * nv, 256)), dim3(256), 0, st, (const D*)G,
+                     (const double*)cf, F, dl, nv);
+  // B_n += sum_m psi_m F_mn
+  hipLaunchKernelGGL((stern_finish_kernel<T, false>), dim3(ceil_div(N, 256), ceil_div(nv, SJB)), dim3(256), 0, st,
+                     (T*)B, (const T*)nullptr, ldb, (const T*)psi, ldpsi, nv, (const D*)F, (const double*)nullptr, N,
+                     nv);
+  CYX_HIP(hipGetLastError());
+  if (dlambda) {
+    CYX_HIP(hipMemcpyAsync(dlambda, dl, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    CYX_HIP(hipStreamSynchronize(st));
+  }
+  return CYX_OK;
+}
+
+}  // namespace cyx
+
+// A batched BiCGSTAB over ncol columns: the workspace, the caller's blocks,
+// and the captured graph of `graph_iters` iterations.
+struct cyx_stern_plan_s {
+  cyx_ham_s* h = nullptr;
+  int dtype = 0, ncol = 0, nv = 0, nchunk = 1;
+  const void* B = nullptr;
+  void* X = nullptr;
+  int64_t ld = 0;
+  const void* PV = nullptr;
+  int64_t ldv = 0;
+  double tol = 0.0;
+  bool precond = false;
+  bool owns_ws = false;  // workspace from hipMalloc (public plans) or the handle's buffer (one solve)
+  void* ws = nullptr;
+  cyx::SternLayout L;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_iters = 0;
+  hipStream_t cap = nullptr;
+};
+
+namespace cyx {
+
+template <class T>
+static int stern_init(cyx_stern_plan_s* sp, const double* lam, int use_guess) {
+  using D = typename acc_of<T>::type;
+  cyx_ham_s* h = sp->h;
+  const int64_t N = h->N();
+  hipStream_t st = h->ctx->stream;
+  const SternLayout& L = sp->L;
+  const int ncol = sp->ncol, nchunk = sp->nchunk;
   CYX_HIP(hipMemcpyAsync(L.lam, lam, ncol * sizeof(double), hipMemcpyHostToDevice, st));
-  const double* dinv = nullptr;
-  if (precond) {
+  if (sp->precond) {
     hipLaunchKernelGGL(dinv_kernel, dim3(ceil_div(h->g.nr, 256)), dim3(256), 0, st, (const double*)h->c64.rcoef,
                        h->g.nr, L.dinv);
     CYX_HIP(hipGetLastError());
-    dinv = L.dinv;
   }
-  T* x = (T*)X;
+  T* x = (T*)sp->X;
+  // x0 (zero unless a guess is given); v = A x0 (bicgstab's r = b - A x0)
+  if (!use_guess) CYX_HIP(hipMemset2DAsync(x, sp->ld * sizeof(T), 0, N * sizeof(T), ncol, st));
+  CYX_HIP(hipMemcpy2DAsync(L.ph, N * sizeof(T), x, sp->ld * sizeof(T), N * sizeof(T), ncol, hipMemcpyDeviceToDevice,
+                           st));
+  int rc;
+  if ((rc = stern_op<T>(h, sp->dtype, (const T*)L.ph, N, (T*)L.v, N, ncol, (const T*)sp->PV, sp->ldv, sp->nv, L)) !=
+      CYX_OK)
+    return rc;
+  const dim3 gc(nchunk, ncol);
+  hipLaunchKernelGGL(bi_init_kernel<T>, gc, dim3(AB), 0, st, (const T*)sp->B, sp->ld, (T*)L.r, (T*)L.rt, (T*)L.p,
+                     (T*)L.v, N, nchunk, (D*)L.part);
+  hipLaunchKernelGGL(bi_init_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)L.part, nchunk, sp->tol,
+                     (ColState<D>*)L.state);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+// one BiCGSTAB iteration of every active column on the context's stream
+template <class T>
+static int stern_iteration(cyx_stern_plan_s* sp) {
+  using D = typename acc_of<T>::type;
+  cyx_ham_s* h = sp->h;
+  const int64_t N = h->N();
+  hipStream_t st = h->ctx->stream;
+  const SternLayout& L = sp->L;
+  const int ncol = sp->ncol, nchunk = sp->nchunk, dtype = sp->dtype;
+  const double* dinv = sp->precond ? L.dinv : nullptr;
+  T* x = (T*)sp->X;
   T *r = (T*)L.r, *rt = (T*)L.rt, *p = (T*)L.p, *v = (T*)L.v, *ph = (T*)L.ph, *sh = (T*)L.sh, *t = (T*)L.t;
   D* part = (D*)L.part;
   ColState<D>* S = (ColState<D>*)L.state;
-  const T* pv = (const T*)PV;
-  int rc;
-  // x0 (zero unless a guess is given); v = A x0 (bicgstab's r = b - A x0)
-  if (!use_guess) CYX_HIP(hipMemset2DAsync(x, ld * sizeof(T), 0, N * sizeof(T), ncol, st));
-  CYX_HIP(hipMemcpy2DAsync(ph, N * sizeof(T), x, ld * sizeof(T), N * sizeof(T), ncol, hipMemcpyDeviceToDevice, st));
-  if ((rc = stern_op<T>(h, dtype, ph, N, v, N, ncol, pv, ldv, nv, L)) != CYX_OK) return rc;
+  const T* pv = (const T*)sp->PV;
   const dim3 gc(nchunk, ncol), gp(ceil_div(N, 256), ncol);
-  hipLaunchKernelGGL(bi_init_kernel<T>, gc, dim3(AB), 0, st, (const T*)B, ld, r, rt, p, v, N, nchunk, part);
-  hipLaunchKernelGGL(bi_init_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, tol, S);
+  int rc;
+  hipLaunchKernelGGL(bi_p_kernel<T>, gp, dim3(256), 0, st, (const T*)r, p, (const T*)v, ph, dinv, h->g.nr, N,
+                     (const ColState<D>*)S);
+  if ((rc = stern_op<T>(h, dtype, ph, N, v, N, ncol, pv, sp->ldv, sp->nv, L)) != CYX_OK) return rc;
+  hipLaunchKernelGGL((coldot_kernel<T, 1>), gc, dim3(AB), 0, st, (const T*)rt, (const T*)v, (const T*)nullptr,
+                     (const T*)nullptr, N, N, nchunk, part);
+  hipLaunchKernelGGL(bi_alpha_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, S);
+  hipLaunchKernelGGL(bi_s_kernel<T>, gc, dim3(AB), 0, st, r, (const T*)v, sh, dinv, h->g.nr, N, nchunk,
+                     (const ColState<D>*)S, part);
+  hipLaunchKernelGGL(bi_s_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, sp->tol, S);
+  if ((rc = stern_op<T>(h, dtype, sh, N, t, N, ncol, pv, sp->ldv, sp->nv, L)) != CYX_OK) return rc;
+  hipLaunchKernelGGL((coldot_kernel<T, 2>), gc, dim3(AB), 0, st, (const T*)t, (const T*)r, (const T*)t,
+                     (const T*)t, N, N, nchunk, part);
+  hipLaunchKernelGGL(bi_omega_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, S);
+  hipLaunchKernelGGL(bi_x_kernel<T>, gc, dim3(AB), 0, st, x, sp->ld, r, (const T*)rt, (const T*)ph, (const T*)sh,
+                     (const T*)t, N, nchunk, (const ColState<D>*)S, part);
+  hipLaunchKernelGGL(bi_r_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, sp->tol, S);
   CYX_HIP(hipGetLastError());
-  int active = ncol, it = 0;
-  const int poll = 4;
-  while (it < maxit && active > 0) {
-    hipLaunchKernelGGL(bi_p_kernel<T>, gp, dim3(256), 0, st, (const T*)r, p, (const T*)v, ph, dinv, h->g.nr, N,
-                       (const ColState<D>*)S);
-    if ((rc = stern_op<T>(h, dtype, ph, N, v, N, ncol, pv, ldv, nv, L)) != CYX_OK) return rc;
-    hipLaunchKernelGGL((coldot_kernel<T, 1>), gc, dim3(AB), 0, st, (const T*)rt, (const T*)v, (const T*)nullptr,
-                       (const T*)nullptr, N, N, nchunk, part);
-    hipLaunchKernelGGL(bi_alpha_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, S);
-    hipLaunchKernelGGL(bi_s_kernel<T>, gc, dim3(AB), 0, st, r, (const T*)v, sh, dinv, h->g.nr, N, nchunk,
-                       (const ColState<D>*)S, part);
-    hipLaunchKernelGGL(bi_s_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, tol, S);
-    if ((rc = stern_op<T>(h, dtype, sh, N, t, N, ncol, pv, ldv, nv, L)) != CYX_OK) return rc;
-    hipLaunchKernelGGL((coldot_kernel<T, 2>), gc, dim3(AB), 0, st, (const T*)t, (const T*)r, (const T*)t,
-                       (const T*)t, N, N, nchunk, part);
-    hipLaunchKernelGGL(bi_omega_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, S);
-    hipLaunchKernelGGL(bi_x_kernel<T>, gc, dim3(AB), 0, st, x, ld, r, (const T*)rt, (const T*)ph, (const T*)sh,
-                       (const T*)t, N, nchunk, (const ColState<D>*)S, part);
-    hipLaunchKernelGGL(bi_r_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, tol, S);
-    CYX_HIP(hipGetLastError());
-    ++it;
-    if (it % poll == 0 || it == maxit) {
-      hipLaunchKernelGGL(count_active_kernel<D>, dim3(1), dim3(AB), 0, st, (const ColState<D>*)S, ncol, L.count);
-      CYX_HIP(hipMemcpyAsync(&active, L.count, sizeof(int), hipMemcpyDeviceToHost, st));
-      CYX_HIP(hipStreamSynchronize(st));
+  return CYX_OK;
+}
+
+static int stern_iterations(cyx_stern_plan_s* sp, int iters) {
+  int rc = CYX_OK;
+  for (int i = 0; i < iters && rc == CYX_OK; ++i) {
+    switch (sp->dtype) {
+      case CYX_F64: rc = stern_iteration<double>(sp); break;
+      case CYX_F32: rc = stern_iteration<float>(sp); break;
+      case CYX_C128: rc = stern_iteration<cplx<double>>(sp); break;
+      default: rc = stern_iteration<cplx<float>>(sp); break;
     }
   }
+  return rc;
+}
+
+int stern_plan_enqueue(cyx_stern_plan_s* sp, hipStream_t st, int iters) {
+  cyx_ctx_s* c = sp->h->ctx;
+  const hipStream_t saved = c->stream;
+  const bool timing = c->timing;
+  c->stream = st;
+  c->timing = false;
+  const int rc = stern_iterations(sp, iters);
+  c->stream = saved;
+  c->timing = timing;
+  return rc;
+}
+
+cyx_ctx_s* stern_plan_context(const cyx_stern_plan_s* sp) { return sp->h->ctx; }
+
+static void stern_plan_drop_graph(cyx_stern_plan_s* sp) {
+  if (sp->exec) (void)hipGraphExecDestroy(sp->exec);
+  if (sp->graph) (void)hipGraphDestroy(sp->graph);
+  sp->exec = nullptr;
+  sp->graph = nullptr;
+  sp->graph_iters = 0;
+}
+
+// replay `iters` iterations from the plan's graph (captured on first use of
+// that count)
+static int stern_plan_run(cyx_stern_plan_s* sp, int iters) {
+  cyx_ctx_s* c = sp->h->ctx;
+  if (!sp->exec || sp->graph_iters != iters) {
+    if (sp->exec) CYX_HIP(hipStreamSynchronize(c->stream));
+    stern_plan_drop_graph(sp);
+    // capture on a stream of the plan's own (capture is not allowed on the
+    // null stream, which the context may use)
+    if (!sp->cap) CYX_HIP(hipStreamCreateWithFlags(&sp->cap, hipStreamNonBlocking));
+    CYX_HIP(hipStreamBeginCapture(sp->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = stern_plan_enqueue(sp, sp->cap, iters);
+    const hipError_t e = hipStreamEndCapture(sp->cap, &sp->graph);
+    if (rc != CYX_OK) {
+      if (sp->graph) (void)hipGraphDestroy(sp->graph);
+      sp->graph = nullptr;
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return CYX_ERR_HIP;
+    }
+    CYX_HIP(hipGraphInstantiate(&sp->exec, sp->graph, nullptr, nullptr, 0));
+    sp->graph_iters = iters;
+  }
+  PhaseTimer tm(c, PH_STERN);
+  CYX_HIP(hipGraphLaunch(sp->exec, c->stream));
+  return CYX_OK;
+}
+
+template <class D>
+static int stern_status_impl(cyx_stern_plan_s* sp, int* n_active, cyx_stern_info* info, int32_t* col_iters,
+                             double* col_relres, int* breakdown) {
+  hipStream_t st = sp->h->ctx->stream;
+  const int ncol = sp->ncol;
   std::vector<ColState<D>> hs(ncol);
-  CYX_HIP(hipMemcpyAsync(hs.data(), S, ncol * sizeof(ColState<D>), hipMemcpyDeviceToHost, st));
+  CYX_HIP(hipMemcpyAsync(hs.data(), sp->L.state, ncol * sizeof(ColState<D>), hipMemcpyDeviceToHost, st));
   CYX_HIP(hipStreamSynchronize(st));
-  int nconv = 0, maxits = 0, breakdown = 0;
+  int nconv = 0, maxits = 0, brk = 0, act = 0;
   double maxrel = 0.0;
   for (int j = 0; j < ncol; ++j) {
     const double rel = hs[j].bnorm2 > 0 ? std::sqrt(hs[j].rnorm2 / hs[j].bnorm2) : 0.0;
     // a half-step exit leaves r = s, whose norm met the tolerance
-    const bool ok = hs[j].bnorm2 == 0.0 || hs[j].half || rel <= tol;
+    const bool ok = hs[j].bnorm2 == 0.0 || hs[j].half || rel <= sp->tol;
+    act += hs[j].active != 0;
     nconv += ok;
-    breakdown += hs[j].flag == 4 && !ok;
+    brk += hs[j].flag == 4 && !ok;
     maxits = std::max(maxits, hs[j].iters);
     maxrel = std::max(maxrel, rel);
     if (col_iters) col_iters[j] = hs[j].iters;
     if (col_relres) col_relres[j] = rel;
   }
+  if (n_active) *n_active = act;
   if (info) {
     info->max_iterations = maxits;
     info->n_converged = nconv;
     info->max_relres = maxrel;
   }
+  if (breakdown) *breakdown = brk;
+  return CYX_OK;
+}
+
+static int stern_status(cyx_stern_plan_s* sp, int* n_active, cyx_stern_info* info, int32_t* col_iters,
+                        double* col_relres, int* breakdown) {
+  if (sp->dtype == CYX_C128 || sp->dtype == CYX_C64)
+    return stern_status_impl<cplx<double>>(sp, n_active, info, col_iters, col_relres, breakdown);
+  return stern_status_impl<double>(sp, n_active, info, col_iters, col_relres, breakdown);
+}
+
+// active-column count only (4 bytes back instead of the column states)
+template <class D>
+static int stern_active_impl(cyx_stern_plan_s* sp, int* active) {
+  hipStream_t st = sp->h->ctx->stream;
+  hipLaunchKernelGGL(count_active_kernel<D>, dim3(1), dim3(AB), 0, st, (const ColState<D>*)sp->L.state, sp->ncol,
+                     sp->L.count);
+  CYX_HIP(hipMemcpyAsync(active, sp->L.count, sizeof(int), hipMemcpyDeviceToHost, st));
+  CYX_HIP(hipStreamSynchronize(st));
+  return CYX_OK;
+}
+
+static int stern_active(cyx_stern_plan_s* sp, int* active) {
+  if (sp->dtype == CYX_C128 || sp->dtype == CYX_C64) return stern_active_impl<cplx<double>>(sp, active);
+  return stern_active_impl<double>(sp, active);
+}
+
+static size_t stern_plan_layout(cyx_stern_plan_s* sp, char* base) {
+  switch (sp->dtype) {
+    case CYX_F64: return stern_layout<double>(sp->h, sp->dtype, sp->ncol, sp->nv, sp->nchunk, true, true, base, sp->L);
+    case CYX_F32: return stern_layout<float>(sp->h, sp->dtype, sp->ncol, sp->nv, sp->nchunk, true, true, base, sp->L);
+    case CYX_C128:
+      return stern_layout<cplx<double>>(sp->h, sp->dtype, sp->ncol, sp->nv, sp->nchunk, true, true, base, sp->L);
+    default:
+      return stern_layout<cplx<float>>(sp->h, sp->dtype, sp->ncol, sp->nv, sp->nchunk, true, true, base, sp->L);
+  }
+}
+
+static int stern_plan_init(cyx_stern_plan_s* sp, const double* lam, int use_guess) {
+  switch (sp->dtype) {
+    case CYX_F64: return stern_init<double>(sp, lam, use_guess);
+    case CYX_F32: return stern_init<float>(sp, lam, use_guess);
+    case CYX_C128: return stern_init<cplx<double>>(sp, lam, use_guess);
+    default: return stern_init<cplx<float>>(sp, lam, use_guess);
+  }
+}
+
+// fills the plan's fields and workspace (own: hipMalloc, else the handle's
+// Sternheimer buffer) and computes the initial residuals
+static int stern_plan_setup(cyx_stern_plan_s* sp, cyx_ham_s* h, int dtype, const void* B, void* X, int ncol,
+                            int64_t ld, const double* lam, const void* PV, int64_t ldv, int nv, int use_guess,
+                            double tol, int precond, bool own) {
+  sp->h = h;
+  sp->dtype = dtype;
+  sp->ncol = ncol;
+  sp->nv = nv;
+  sp->B = B;
+  sp->X = X;
+  sp->ld = ld;
+  sp->PV = PV;
+  sp->ldv = ldv;
+  sp->tol = tol;
+  sp->precond = precond != 0;
+  sp->nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(128, ceil_div(h->N(), 2048)));
+  const size_t bytes = stern_plan_layout(sp, nullptr);
+  if (own) {
+    CYX_CHECK(hipMalloc(&sp->ws, bytes) == hipSuccess, "Sternheimer plan workspace allocation failed (%zu bytes)",
+              bytes);
+    sp->owns_ws = true;
+  } else {
+    CYX_CHECK(ensure(h->stern, bytes) == CYX_OK, "Sternheimer workspace allocation failed");
+    sp->ws = h->stern.p;
+  }
+  stern_plan_layout(sp, (char*)sp->ws);
+  return stern_plan_init(sp, lam, use_guess);
+}
+
+static void stern_plan_release(cyx_stern_plan_s* sp) {
+  // a replay may still run
+  if (sp->exec) (void)hipStreamSynchronize(sp->h->ctx->stream);
+  stern_plan_drop_graph(sp);
+  if (sp->cap) (void)hipStreamDestroy(sp->cap);
+  sp->cap = nullptr;
+  if (sp->owns_ws) (void)hipFree(sp->ws);
+  sp->ws = nullptr;
+}
+
+// iterations in graph replays of `poll` iterations, the active count read
+// after each (the previous host-polled loop's cadence)
+static int stern_solve_impl(cyx_ham_s* h, int dtype, const void* B, void* X, int ncol, int64_t ld, const double* lam,
+                            const void* PV, int64_t ldv, int nv, int use_guess, double tol, int maxit, int precond,
+                            cyx_stern_info* info, int32_t* col_iters, double* col_relres) {
+  cyx_stern_plan_s sp;
+  int rc = stern_plan_setup(&sp, h, dtype, B, X, ncol, ld, lam, PV, ldv, nv, use_guess, tol, precond, false);
+  int active = ncol, it = 0;
+  const int poll = 4;
+  if (rc == CYX_OK) rc = stern_active(&sp, &active);
+  while (rc == CYX_OK && it < maxit && active > 0) {
+    const int k = std::min(poll, maxit - it);
+    // a short final round runs without a graph of its own
+    rc = k == poll ? stern_plan_run(&sp, k) : stern_plan_enqueue(&sp, h->ctx->stream, k);
+    it += k;
+    if (rc == CYX_OK) rc = stern_active(&sp, &active);
+  }
+  int breakdown = 0;
+  cyx_stern_info inf{};
+  if (rc == CYX_OK) rc = stern_status(&sp, nullptr, &inf, col_iters, col_relres, &breakdown);
+  stern_plan_release(&sp);
+  if (rc != CYX_OK) return rc;
+  if (info) *info = inf;
   if (breakdown) {
     set_error("BiCGSTAB broke down (rho or omega zero) on %d of %d columns", breakdown, ncol);
     return CYX_ERR_NOCONV;
   }
-  if (nconv < ncol) {
-    set_error("BiCGSTAB reached %d iterations with %d of %d columns above tol %g", maxit, ncol - nconv, ncol, tol);
+  if (inf.n_converged < ncol) {
+    set_error("BiCGSTAB reached %d iterations with %d of %d columns above tol %g", maxit, ncol - inf.n_converged,
+              ncol, tol);
     return CYX_ERR_NOCONV;
   }
   return CYX_OK;
@@ -532,8 +817,8 @@ static int stern_check(cyx_ham_t h, int dtype, int ncol, int64_t ld, const doubl
   CYX_CHECK(h && lam, "null argument");
   CYX_CHECK(dtype == CYX_F64 || dtype == CYX_C128 || dtype == CYX_F32 || dtype == CYX_C64, "bad dtype %d", dtype);
   CYX_CHECK(!h->bloch || dtype == CYX_C128 || dtype == CYX_C64, "a Bloch channel needs a complex dtype");
-  CYX_CHECK(ncol >= 1 && ld >= h->N(), "bad block shape (ncol=%d ld=%lld N=%lld)", ncol, (long long)ld,
-            (long long)h->N());
+  CYX_CHECK(ncol >= 1 && ncol <= 65535 && ld >= h->N(), "bad block shape (ncol=%d ld=%lld N=%lld)", ncol,
+            (long long)ld, (long long)h->N());
   CYX_CHECK(nv >= 0 && (nv == 0 || (PV && ldv >= h->N())), "bad projector block (nv=%d)", nv);
   return CYX_OK;
 }
@@ -560,19 +845,61 @@ int cyx_sternheimer_solve(cyx_ham_t h, int dtype, const void* B, void* X, int nc
   if ((rc = stern_check(h, dtype, ncol, ld, lam, psi_v, ldv, nv)) != CYX_OK) return rc;
   CYX_CHECK(B && X && B != X, "null or aliased argument");
   CYX_CHECK(tol > 0 && max_iter >= 1, "bad tolerance or iteration limit");
+  return stern_solve_impl(h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol, max_iter, precond, info,
+                          col_iters, col_relres);
+}
+
+int cyx_sternheimer_plan_create(cyx_ham_t h, int dtype, const void* B, void* X, int ncol, int64_t ld,
+                                const double* lam, const void* psi_v, int64_t ldv, int nv, int use_guess, double tol,
+                                int precond, cyx_stern_plan_t* out) {
+  int rc;
+  CYX_CHECK(out, "null argument");
+  *out = nullptr;
+  if ((rc = stern_check(h, dtype, ncol, ld, lam, psi_v, ldv, nv)) != CYX_OK) return rc;
+  CYX_CHECK(B && X && B != X, "null or aliased argument");
+  CYX_CHECK(tol > 0, "bad tolerance");
+  cyx_stern_plan_s* sp = new cyx_stern_plan_s();
+  rc = stern_plan_setup(sp, h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol, precond, true);
+  if (rc != CYX_OK) {
+    (void)hipStreamSynchronize(h->ctx->stream);
+    stern_plan_release(sp);
+    delete sp;
+    return rc;
+  }
+  *out = sp;
+  return CYX_OK;
+}
+
+int cyx_sternheimer_plan_iterate(cyx_stern_plan_t sp, int iters) {
+  CYX_CHECK(sp && iters >= 1, "bad argument");
+  return stern_plan_run(sp, iters);
+}
+
+int cyx_sternheimer_plan_status(cyx_stern_plan_t sp, int* n_active, cyx_stern_info* info, int32_t* col_iters,
+                                double* col_relres) {
+  CYX_CHECK(sp, "null argument");
+  return stern_status(sp, n_active, info, col_iters, col_relres, nullptr);
+}
+
+int cyx_sternheimer_plan_destroy(cyx_stern_plan_t sp) {
+  if (!sp) return CYX_OK;
+  (void)hipDeviceSynchronize();  // the context may already be gone
+  stern_plan_release(sp);
+  delete sp;
+  return CYX_OK;
+}
+
+int cyx_sternheimer_rhs(cyx_ham_t dham, int dtype, const void* psi, int64_t ldpsi, int nv, const double* coeff,
+                        void* B, int64_t ldb, double* dlambda) {
+  CYX_CHECK(dham && psi && coeff && B && psi != B, "null or aliased argument");
+  CYX_CHECK(dtype == CYX_F64 || dtype == CYX_C128 || dtype == CYX_F32 || dtype == CYX_C64, "bad dtype %d", dtype);
+  CYX_CHECK(!dham->bloch || dtype == CYX_C128 || dtype == CYX_C64, "a Bloch channel needs a complex dtype");
+  CYX_CHECK(nv >= 1 && nv <= 65535 && ldpsi >= dham->N() && ldb >= dham->N(), "bad block shape (nv=%d)", nv);
   switch (dtype) {
-    case CYX_F64:
-      return stern_solve_impl<double>(h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol, max_iter,
-                                      precond, info, col_iters, col_relres);
-    case CYX_F32:
-      return stern_solve_impl<float>(h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol, max_iter,
-                                     precond, info, col_iters, col_relres);
-    case CYX_C128:
-      return stern_solve_impl<cplx<double>>(h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol,
-                                            max_iter, precond, info, col_iters, col_relres);
-    default:
-      return stern_solve_impl<cplx<float>>(h, dtype, B, X, ncol, ld, lam, psi_v, ldv, nv, use_guess, tol, max_iter,
-                                           precond, info, col_iters, col_relres);
+    case CYX_F64: return stern_rhs_impl<double>(dham, dtype, psi, ldpsi, nv, coeff, B, ldb, dlambda);
+    case CYX_F32: return stern_rhs_impl<float>(dham, dtype, psi, ldpsi, nv, coeff, B, ldb, dlambda);
+    case CYX_C128: return stern_rhs_impl<cplx<double>>(dham, dtype, psi, ldpsi, nv, coeff, B, ldb, dlambda);
+    default: return stern_rhs_impl<cplx<float>>(dham, dtype, psi, ldpsi, nv, coeff, B, ldb, dlambda);
   }
 }
 

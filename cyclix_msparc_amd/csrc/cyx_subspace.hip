@@ -383,50 +383,86 @@ __global__ void hermitize_kernel(D* M, int ns) {
 }
 
 // ---------------------------------------------------------------- host side
+struct GramShape {
+  int nta, ntp, nchb;
+  int64_t chunk;
+  size_t pbytes;
+};
+
 template <class T>
-static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-                     const double* W, void* M_out, bool herm) {
+static GramShape gram_shape(int64_t N, int na, int nb, bool herm) {
   using D = typename acc_of<T>::type;
-  const int64_t N = h->N();
-  herm = herm && na == nb;
-  const int nta = ceil_div(na, GS), ntb = ceil_div(nb, GS);
-  const int ntp = herm ? nta * (nta + 1) / 2 : nta * ntb;
+  GramShape g;
+  g.nta = ceil_div(na, GS);
+  const int ntb = ceil_div(nb, GS);
+  g.ntp = herm ? g.nta * (g.nta + 1) / 2 : g.nta * ntb;
   // about 8192 waves in all (4 per block; 16384 for a Hermitian product,
   // whose 36 of 64 tiles leave the machine short of waves otherwise:
   // measured 4.3 against 4.46 ms per step for both products, with 4096 or
   // 32768 slower), k sub-ranges in whole k steps
-  int nchb = std::max(1, ceil_div(herm ? 16384 : 8192, 4 * ntp));
+  int nchb = std::max(1, ceil_div(herm ? 16384 : 8192, 4 * g.ntp));
   const int kstep = 4 * gram_sp<T>();
   int64_t chunk = ceil_div(N, 4 * (int64_t)nchb);
   chunk = ceil_div(chunk, kstep) * kstep;
-  nchb = (int)ceil_div(N, 4 * chunk);
-  const size_t pbytes = (size_t)ntp * nchb * GS * GS * sizeof(D);
-  CYX_CHECK(ensure(h->small, pbytes) == CYX_OK, "partials allocation failed");
+  g.nchb = (int)ceil_div(N, 4 * chunk);
+  g.chunk = chunk;
+  g.pbytes = (size_t)g.ntp * g.nchb * GS * GS * sizeof(D);
+  return g;
+}
+
+size_t gram_part_bytes(int dtype, int64_t N, int na, int nb, bool herm) {
+  herm = herm && na == nb;
+  switch (dtype) {
+    case CYX_F64:
+      return gram_shape<double>(N, na, nb, herm).pbytes;
+    case CYX_F32:
+      return gram_shape<float>(N, na, nb, herm).pbytes;
+    case CYX_C128:
+      return gram_shape<cplx<double>>(N, na, nb, herm).pbytes;
+    case CYX_C64:
+      return gram_shape<cplx<float>>(N, na, nb, herm).pbytes;
+  }
+  return 0;
+}
+
+template <class T>
+static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                     const double* W, void* M_out, bool herm, void* part) {
+  using D = typename acc_of<T>::type;
+  const int64_t N = h->N();
+  herm = herm && na == nb;
+  const GramShape gs = gram_shape<T>(N, na, nb, herm);
+  const int nta = gs.nta, ntp = gs.ntp, nchb = gs.nchb;
+  const int64_t chunk = gs.chunk;
+  if (!part) {
+    CYX_CHECK(ensure(h->small, gs.pbytes) == CYX_OK, "partials allocation failed");
+    part = h->small.p;
+  }
   hipStream_t st = h->ctx->stream;
   PhaseTimer tm(h->ctx, PH_GRAM);
   // 16-byte vector loads when every lane's 32-byte run of a column is aligned
   const auto al = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   const int vec = ((lda * sizeof(T)) % 16 == 0 && (ldb * sizeof(T)) % 16 == 0 && al(A) && al(B)) ? 1 : 0;
   hipLaunchKernelGGL(gram_mfma_kernel<T>, dim3(ntp, nchb), dim3(256), 0, st, (const T*)A, lda, na, (const T*)B, ldb,
-                     nb, W, N, chunk, herm ? 1 : 0, nta, vec, (D*)h->small.p);
+                     nb, W, N, chunk, herm ? 1 : 0, nta, vec, (D*)part);
   CYX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / 256), dim3(256), 0, st, (const D*)h->small.p, nchb,
+  hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / 256), dim3(256), 0, st, (const D*)part, nchb,
                      herm ? 1 : 0, nta, na, nb, (D*)M_out);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
 }
 
 int gram_rect(cyx_ham_s* h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
-              const double* W, void* M_out, bool herm) {
+              const double* W, void* M_out, bool herm, void* part) {
   switch (dtype) {
     case CYX_F64:
-      return gram_impl<double>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
+      return gram_impl<double>(h, A, lda, na, B, ldb, nb, W, M_out, herm, part);
     case CYX_F32:
-      return gram_impl<float>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
+      return gram_impl<float>(h, A, lda, na, B, ldb, nb, W, M_out, herm, part);
     case CYX_C128:
-      return gram_impl<cplx<double>>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
+      return gram_impl<cplx<double>>(h, A, lda, na, B, ldb, nb, W, M_out, herm, part);
     case CYX_C64:
-      return gram_impl<cplx<float>>(h, A, lda, na, B, ldb, nb, W, M_out, herm);
+      return gram_impl<cplx<float>>(h, A, lda, na, B, ldb, nb, W, M_out, herm, part);
   }
   return CYX_ERR_ARG;
 }

@@ -162,8 +162,29 @@ class Species:
         out[~small] = -self.Z * erf(dd / (math.sqrt(2) * self.sigma_loc)) / dd
         return out
 
+    def dvloc_over_d(self, d):
+        """V_loc'(d) / d, finite at d = 0 (the radial derivative of the local
+        potential divided by the distance, for the displacement derivative
+        dV/dX = -V'(d)/d (x - X))."""
+        d = np.asarray(d, dtype=float)
+        s = math.sqrt(2) * self.sigma_loc
+        x = d / s
+        out = np.empty_like(d)
+        small = x < 0.05
+        # series of erf(x)/x: V'/d = (Z/s^3)(2/sqrt(pi))(2/3 - 2x^2/5 + x^4/7 - x^6/27)
+        xs = x[small] ** 2
+        out[small] = (self.Z / s ** 3) * (2 / math.sqrt(math.pi)) * (2 / 3 - xs * (2 / 5 - xs * (1 / 7 - xs / 27)))
+        dd = d[~small]
+        g = math.sqrt(2 / math.pi) / self.sigma_loc * np.exp(-dd ** 2 / (2 * self.sigma_loc ** 2))
+        out[~small] = -self.Z * (g / dd ** 2 - erf(dd / s) / dd ** 3)
+        return out
+
     def udv(self, d):
         return np.exp(-np.asarray(d) ** 2 / (2 * self.sigma_nl ** 2))
+
+    def dudv_over_d(self, d):
+        """UdV'(d) / d of the Gaussian projector radial form."""
+        return -self.udv(d) / self.sigma_nl ** 2
 
     def rho_iso(self, d):
         s = self.sigma_rho
@@ -220,6 +241,15 @@ class CyclixSystem:
         rby = math.ceil(rby / g.dt - 1e-12) * g.dt
         return rbx, rby, rbz
 
+    def image_rotation(self, J, im):
+        """Cartesian rotation taking atom J's displacement to that of its image
+        at ``im`` (r, theta, z): the image angle offset (theta_img - theta_J) +
+        twist (z_img - z_J) about the axis.  The reference's
+        ``S.RotM1^fac1 * S.RotM2^fac2`` (``src/phonon.m:1381-1386``)."""
+        a = (im[1] - self.atoms[J, 1]) + self.grid.twist * (im[2] - self.atoms[J, 2])
+        c, s_ = math.cos(a), math.sin(a)
+        return np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+
     def _images(self, x0):
         """Periodic images in theta and z.  ``src/calculate_nloc_projector.m:50-81``."""
         g = self.grid
@@ -271,6 +301,35 @@ class CyclixSystem:
                 chi = (sp.udv(d[ball]) * Y00)[:, None]
                 A.images.append(ProjImage(rc_pos=rc_pos, chi=chi, coords=np.array(im, dtype=float)))
             out.append(A)
+        return out
+
+    def projector_derivatives(self, u, proj=None):
+        """First-order change of every projector image under the Cartesian
+        atom displacements u (n_atm x 3): dchi = grad_X chi . (R_img u_J), on
+        the node set of ``projectors()`` (exact derivative of the discrete
+        tables, whose node sets do not move for small displacements).  The
+        reference forms the same term from the finite-difference gradient of
+        Chi_full (``src/phonon.m:1365-1399``, DChi = -dchi); here the radial
+        form is differentiated in closed form.  Returns a list per atom of
+        arrays per image (n_nodes, angnum)."""
+        g = self.grid
+        sp = self.species
+        proj = self.projectors() if proj is None else proj
+        u = np.asarray(u, dtype=float).reshape(self.n_atm, 3)
+        out = []
+        for J, A in enumerate(proj):
+            per = []
+            for im in A.images:
+                q = im.rc_pos.astype(np.int64)
+                ii, jj, kk = q % g.nr, (q // g.nr) % g.nt, q // (g.nr * g.nt)
+                pts = np.stack([g.xin + ii * g.dr, jj * g.dt, kk * g.dz], axis=1)
+                diff = to_cart(g, pts) - to_cart(g, im.coords)
+                d = np.linalg.norm(diff, axis=1)
+                v = self.image_rotation(J, im.coords) @ u[J]
+                # d chi / d X_img = -UdV'(d)/d (x - X_img) Y00
+                dchi = -(sp.dudv_over_d(d) * Y00) * (diff @ v)
+                per.append(dchi[:, None] * np.ones((1, im.chi.shape[1])))
+            out.append(per)
         return out
 
     # ------------------------------------------------------------------
@@ -359,6 +418,46 @@ class CyclixSystem:
         E_corr = 0.5 * np.sum((b_ref + b) * V_c * W) + Eself - Eself_ref
         return dict(b=b, b_ref=b_ref, Eself=Eself, E_corr=E_corr, rho_at=rho_at,
                     PosCharge=pos, NegCharge=neg)
+
+    def pseudocharge_derivative(self, u):
+        """First-order change of the pseudocharge b under the Cartesian atom
+        displacements u (n_atm x 3): the box Laplacian of
+        dV_J = grad_X V_J . (R_img u_J) on the boxes of ``pseudocharge()``
+        (the reference's S.Db_dR of ``electrostatics_local``,
+        ``src/phonon.m:2164-2330``, for one displacement pattern)."""
+        g = self.grid
+        sp = self.species
+        u = np.asarray(u, dtype=float).reshape(self.n_atm, 3)
+        db = np.zeros(g.N)
+        rbx, rby, rbz = self.rb_xyz()
+        n0 = g.fdn
+        for J in range(self.n_atm):
+            for im in self._images(self.atoms[J]):
+                x0i, y0i, z0i = im
+                i_s = max(math.ceil(((x0i - g.xin) - rbx) / g.dr) + 1, 1)
+                i_e = min(math.floor(((x0i - g.xin) + rbx) / g.dr) + 1, g.nr)
+                j_s = max(math.ceil((y0i - rby) / g.dt) + 1, 1)
+                j_e = min(math.floor((y0i + rby) / g.dt) + 1, g.nt)
+                k_s = max(math.ceil((z0i - rbz) / g.dz) + 1, 1)
+                k_e = min(math.floor((z0i + rbz) / g.dz) + 1, g.nz)
+                if i_e < i_s or j_e < j_s or k_e < k_s:
+                    continue
+                xx = g.xin + np.arange(i_s - n0 - 1, i_e + n0) * g.dr
+                yy = np.arange(j_s - n0 - 1, j_e + n0) * g.dt
+                zz = np.arange(k_s - n0 - 1, k_e + n0) * g.dz
+                XX, YY, ZZ = np.meshgrid(xx, yy, zz, indexing="ij")
+                pts = np.stack([XX.ravel(), YY.ravel(), ZZ.ravel()], axis=1)
+                diff = to_cart(g, pts) - to_cart(g, im)
+                dd = np.linalg.norm(diff, axis=1)
+                v = self.image_rotation(J, im) @ u[J]
+                # dV/dX_img = -V'(d)/d (x - X_img)
+                dV = (-sp.dvloc_over_d(dd) * (diff @ v)).reshape(XX.shape)
+                dbJ = self._box_laplacian(dV, xx[0])
+                IIb, JJb, KKb = np.meshgrid(np.arange(i_s, i_e + 1), np.arange(j_s, j_e + 1),
+                                            np.arange(k_s, k_e + 1), indexing="ij")
+                rows = ((KKb - 1) * g.nr * g.nt + (JJb - 1) * g.nr + IIb - 1).ravel()
+                np.add.at(db, rows, dbJ.ravel())
+        return db * (-1 / (4 * math.pi))
 
 
 # --------------------------------------------------------------------------

@@ -80,6 +80,7 @@ typedef struct cyx_poisson_s* cyx_poisson_t;
 typedef struct cyx_mixer_s* cyx_mixer_t;
 typedef struct cyx_unet_s* cyx_unet_t;
 typedef struct cyx_filter_plan_s* cyx_filter_plan_t;
+typedef struct cyx_stern_plan_s* cyx_stern_plan_t;
 
 /* ------------------------------------------------------------ context */
 const char* cyx_last_error(void);
@@ -97,7 +98,8 @@ int cyx_ctx_synchronize(cyx_ctx_t ctx);
  * three-term update: first filter step, Lanczos), "kb_gather" (KB gathers
  * and scatters), "gram", "rotate", "wnorm", "dense_eig", "aar" (one whole
  * AAR solve), "rr_apply" (the FP64-arithmetic H psi of Rayleigh-Ritz and of
- * the residual check).  The reference prints only tic/toc wall times
+ * the residual check), "sternheimer" (one replay of a Sternheimer plan's
+ * captured iterations).  The reference prints only tic/toc wall times
  * (src/scf.m:284-304, src/poissonSolve.m:49). */
 int cyx_ctx_set_timing(cyx_ctx_t ctx, int enable);
 int cyx_ctx_reset_timing(cyx_ctx_t ctx);
@@ -207,6 +209,15 @@ int cyx_filter_plan_create(cyx_ctx_t ctx, int nch, const cyx_ham_t* hams, int dt
 int cyx_filter_plan_run(cyx_filter_plan_t plan, const double* a, const double* b, const double* a0,
                         int* result_in_b);
 int cyx_filter_plan_tuning(cyx_filter_plan_t plan, cyx_tuning* t);
+/* Puts `iters` iterations of a Sternheimer plan (declared further down,
+ * same context) into the plan's graph beside the filter steps, as an
+ * independent branch: every cyx_filter_plan_run then filters all channels
+ * and advances all right-hand sides in one graph launch (DFPT solves
+ * batched with the CheFSI channels).  sp = NULL detaches.  The Sternheimer
+ * plan must outlive the attachment; poll it with
+ * cyx_sternheimer_plan_status.  No counterpart in the reference, which runs
+ * the two loops one after the other on the host. */
+int cyx_filter_plan_attach_sternheimer(cyx_filter_plan_t plan, cyx_stern_plan_t sp, int iters);
 int cyx_filter_plan_destroy(cyx_filter_plan_t plan);
 
 /* ------------------------------------------------------------ SCF glue (FP64) */
@@ -269,6 +280,41 @@ typedef struct cyx_stern_info {
 int cyx_sternheimer_solve(cyx_ham_t ham, int dtype, const void* B, void* X, int ncol, int64_t ld, const double* lam,
                           const void* psi_v, int64_t ldv, int nv, int use_guess, double tol, int max_iter,
                           int precond, cyx_stern_info* info, int32_t* col_iters, double* col_relres);
+
+/* The same solve as a resident plan whose iterations are captured in a HIP
+ * graph: create (workspace, x0, r = b - A x0), then replay `iters`
+ * iterations per call until cyx_sternheimer_plan_status reports no active
+ * column.  Converged columns freeze on the device (their x and r stay), so a
+ * replay never needs the host.  B, X, lam's columns and psi_v must stay
+ * allocated and in place for the plan's life (the graph holds their
+ * addresses).  Replaces the same per-state bicgstab loop
+ * (src/phonon.m:1516-1521); cyx_sternheimer_solve runs on it. */
+int cyx_sternheimer_plan_create(cyx_ham_t ham, int dtype, const void* B, void* X, int ncol, int64_t ld,
+                                const double* lam, const void* psi_v, int64_t ldv, int nv, int use_guess, double tol,
+                                int precond, cyx_stern_plan_t* out);
+int cyx_sternheimer_plan_iterate(cyx_stern_plan_t sp, int iters);
+/* Synchronises; n_active: columns still iterating.  info, col_iters and
+ * col_relres as for cyx_sternheimer_solve (may be NULL). */
+int cyx_sternheimer_plan_status(cyx_stern_plan_t sp, int* n_active, cyx_stern_info* info, int32_t* col_iters,
+                                double* col_relres);
+int cyx_sternheimer_plan_destroy(cyx_stern_plan_t sp);
+
+/* Right-hand sides of the Sternheimer equations of one perturbation
+ * (src/phonon.m:1358-1444, "Method-2", q = 0):
+ *   dH psi_n = dVeff .* psi_n + sum_J gamma_J (chi_J dchi_J^H + dchi_J chi_J^H) W psi_n
+ *   G = psi^H W dH psi                       (psi_DH_psi, nv x nv)
+ *   B_n = sum_m psi_m G_mn coeff_mn - dH psi_n
+ *   dlambda_n = G_nn                         (S.Dlambda_dR)
+ * `dham` is a handle of the same grid and channel whose potential is dVeff
+ * (S.Dveff_dR(:,scf_natmc)) and whose projectors carry the projector
+ * derivatives as the pairs (chi +- dchi) / sqrt(2) with gammas +-gamma_J
+ * (their KB term is exactly the first-order nonlocal term above); its
+ * kinetic stencil is not used.  psi: device, nv states (ldpsi); B: device,
+ * nv columns (ldb); coeff: host, nv x nv column-major, the reference's
+ * S.coeff_Pv_psi_kq(1:nv,1:nv,kpt) after its transpose (src/phonon.m:112-158);
+ * dlambda: host, nv (may be NULL). */
+int cyx_sternheimer_rhs(cyx_ham_t dham, int dtype, const void* psi, int64_t ldpsi, int nv, const double* coeff,
+                        void* B, int64_t ldb, double* dlambda);
 
 /* ------------------------------------------------------------ Poisson */
 typedef struct cyx_aar_info {

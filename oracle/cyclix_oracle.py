@@ -730,6 +730,57 @@ def sternheimer_apply(ham: Hamiltonian, lam, psi_v, X):
     return HX
 
 
+def coeff_pv_psi(eigval, occ, bet, nv):
+    """S.coeff_Pv_psi_kq(1:nv,1:nv,kpt) for q = 0 (EigVal_kq = EigVal, occ_kq =
+    occ, S.Kdeltaq0 = 1), after the transpose.  ``src/phonon.m:116-158``."""
+    c = np.zeros((nv, nv))
+    for i in range(nv):
+        for j in range(nv):
+            temp = eigval[i] - eigval[j]
+            temp_occ = occ[i] - occ[j]
+            if j != i:
+                if abs(temp) < 1e-4 and abs(temp_occ) < 1e-4:
+                    c[i, j] = 1 - (1 - occ[i]) * bet / 2
+                else:
+                    c[i, j] = 1 / temp
+            else:
+                c[i, j] = 1.0
+    return c.T.copy()
+
+
+def sternheimer_rhs(W, kpt, dveff, atoms, datoms, psi, coeff):
+    """Right-hand sides of one perturbation, ``src/phonon.m:1358-1444``
+    ("Method-2"): PsiW, DVeff_psi, Chi_full and DChi per atom (DChi is the
+    gradient of Chi_full with respect to the field point, i.e. minus the
+    derivative with respect to the atom position held in ``datoms``),
+    integral_1, integral_2, psi_DH_psi and rhs.  psi: N x nv.  Returns
+    (rhs N x nv, dlambda = diag(psi_DH_psi))."""
+    kpt = np.asarray(kpt, dtype=float)
+    fac = 1.0j
+    N, nv = psi.shape
+    PsiW = (np.conj(psi) * W[:, None]).T
+    DVeff_psi = dveff[:, None] * psi
+    psi_DVeff_psi = PsiW @ DVeff_psi
+    psi_DVnl_psi = np.zeros((nv, nv), dtype=complex)
+    rhs_nl = np.zeros((N, nv), dtype=complex)
+    for A, dA in zip(atoms, datoms):
+        ang = A.gamma.size
+        Chi_full = np.zeros((N, ang), dtype=complex)
+        DChi = np.zeros((N, ang), dtype=complex)
+        for im, dim in zip(A.images, dA.images):
+            phase_fac = np.exp(-np.dot(kpt, (A.coords - im.coords) * fac))
+            Chi_full[im.rc_pos, :] += im.chi * phase_fac
+            DChi[im.rc_pos, :] += -dim.chi * phase_fac
+        integral_1 = np.conj(PsiW) @ np.conj(Chi_full)
+        integral_2 = -PsiW @ DChi
+        temp = integral_2 @ (integral_1.T * A.gamma[:, None])
+        psi_DVnl_psi += temp + temp.conj().T
+        rhs_nl += -Chi_full @ (integral_2.conj().T * A.gamma[:, None]) + DChi @ (integral_1.T * A.gamma[:, None])
+    psi_DH_psi = psi_DVeff_psi + psi_DVnl_psi
+    rhs = psi @ (psi_DH_psi * coeff) + rhs_nl - DVeff_psi
+    return rhs, np.real(np.diag(psi_DH_psi))
+
+
 # --------------------------------------------------------------------------
 # SCF loop
 # --------------------------------------------------------------------------
