@@ -72,6 +72,8 @@ def parse():
                         "weights exist")
     p.add_argument("--no-learned", action="store_true", help="skip everything learned (no weights loaded)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-batched", action="store_true",
+                   help="skip the batched workload (32 channels on two grids + 48 Sternheimer right-hand sides)")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
     p.add_argument("--poisson-only", action="store_true",
@@ -196,6 +198,131 @@ def run_path(dev, sysm, args, precision, unet=None):
     return rec, state
 
 
+def batched_workload(dev, gs, npl):
+    """The batched workload: 32 helical Bloch channels (axial Bloch
+    wavenumbers eta_c = 2 pi c / (32 L3)) with state counts drawn uniformly
+    from 64..320 (seed 1), alternating between the 120x24x96 and 80x24x64
+    grids of the (10,0) sector, filtered in complex64 at degree npl by one
+    autotuned plan; and the Sternheimer solves of 48 right-hand sides from
+    random displacement patterns (seed 1) on the converged FP64 ground state
+    of the 120x24x96 channel (complex128, bicgstab tol 1e-8).  Timed: a filter
+    pass of all channels, the 48 solves alone (graph replays), and both in one
+    graph launch per pass (the Sternheimer iterations an independent branch
+    of the filter pass's graph) until every right-hand side has converged."""
+    import torch
+
+    from cyclix_msparc_amd import dfpt
+    from cyclix_msparc_amd.api import FilterPlan, Hamiltonian, Poisson, SternheimerPlan, lda_pw
+    from cyclix_msparc_amd.system import make_system
+    tdev = dev.torch_device
+    systems = [gs.system, make_system("cnt10_helical_80x64")]
+    # channel potentials: the converged SCF potential on the 120x24x96 grid,
+    # the initial SCF potential (src/scf.m:32-39) on the 80x24x64 grid
+    sB = systems[1]
+    pcB = sB.pseudocharge()
+    rhoB = torch.from_numpy(pcB["rho_at"]).to(tdev)
+    poisB = Poisson(dev, sB.grid)
+    phiB, _ = poisB.solve(rhoB, torch.from_numpy(pcB["b"]).to(tdev), None, 1e-8, 3000)
+    _, veffB = lda_pw(dev, rhoB, phiB)
+    poisB.close()
+    veffs = [gs.veff, veffB]
+    projs = [s_.projectors() for s_ in systems]
+    rng = np.random.default_rng(1)
+    nch = 32
+    ncols = rng.integers(64, 321, size=nch)
+    hams, blocks = [], []
+    torch.manual_seed(1)
+    for c in range(nch):
+        s_ = systems[c % 2]
+        hams.append(Hamiltonian(dev, s_, (0.0, 0.0, 2 * math.pi * c / (nch * s_.grid.L3)), veffs[c % 2],
+                                projs[c % 2]))
+        blocks.append(torch.randn((int(ncols[c]), s_.grid.N), dtype=torch.complex64, device=tdev))
+    bnd = []
+    for k in range(2):
+        lo, hi = hams[k].lanczos_bounds(rng.random(systems[k].grid.N), 300, 1e-2)
+        bnd.append((lo + 0.25 * (hi - lo), 1.01 * hi, lo - 0.1))
+    a, b, a0 = (np.array([bnd[c % 2][i] for c in range(nch)]) for i in range(3))
+    plan = FilterPlan(dev, hams, blocks, npl, autotune=True, bounds=(a, b, a0))
+    plan.run(a, b, a0)
+    torch.cuda.synchronize()
+    npass = 3
+    t = time.perf_counter()
+    for _ in range(npass):
+        plan.run(a, b, a0)
+    torch.cuda.synchronize()
+    filt_ms = (time.perf_counter() - t) * 1e3 / npass
+    tuning = plan.tuning()
+    # algorithmic bytes of a pass, as for the single-channel filter:
+    # (3m - 1) N ns b + m N 8 per channel (b = 8 for complex64)
+    Ns = [systems[c % 2].grid.N for c in range(nch)]
+    b_pass = sum((3 * npl - 1) * Ns[c] * int(ncols[c]) * 8.0 + npl * Ns[c] * 8.0 for c in range(nch))
+    filt_gbs = b_pass / (filt_ms * 1e-3) / 1e9
+    # the Sternheimer right-hand sides and solves
+    res = gs.result
+    occ = res.occ[:, 0]
+    nv = dfpt.occupied_count(occ)
+    npert = math.ceil(48 / nv)
+    U = dfpt.random_displacements(gs.system.n_atm, npert, seed=1)
+    psi = gs.psi[0]
+    t = time.perf_counter()
+    B, lam, dl, _ = dfpt.sternheimer_rhs_batch(dev, gs.system, gs.kpts[0], psi, res.EigVal[:, 0], occ, gs.opt.bet,
+                                               gs.poisson, U, proj=projs[0])
+    torch.cuda.synchronize()
+    rhs_ms = (time.perf_counter() - t) * 1e3
+    B, lam = B[:48].contiguous(), lam[:48]
+    poll = 8
+    sp = SternheimerPlan(gs.hams[0], B, lam, psi[:nv], tol=1e-8)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    act, info, its, relres = sp.solve(max_iter=3000, poll=poll)
+    torch.cuda.synchronize()
+    stern_ms = (time.perf_counter() - t) * 1e3
+    sp.close()
+    it_run = math.ceil(info.max_iterations / poll) * poll
+    ms_it = stern_ms / max(it_run, 1)
+    # one graph launch per pass: the filter steps and k iterations, k sized so
+    # that the Sternheimer branch lasts about as long as the filter pass
+    k = max(4, int(round(filt_ms / ms_it)))
+    sp2 = SternheimerPlan(gs.hams[0], B, lam, psi[:nv], tol=1e-8)
+    plan.attach_sternheimer(sp2, k)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    launches = 0
+    while True:
+        plan.run(a, b, a0)
+        launches += 1
+        act2 = sp2.status()[0]
+        if act2 == 0 or launches >= 400:
+            break
+    torch.cuda.synchronize()
+    comb_ms = (time.perf_counter() - t) * 1e3
+    act2, info2, its2, rel2 = sp2.status()
+    plan.attach_sternheimer(None)
+    sp2.close()
+    plan.close()
+    for h in hams:
+        h.close()
+    del blocks
+    return {
+        "workload": f"32 helical channels (eta_c = 2 pi c / (32 L3)), states drawn from 64..320 (seed 1), "
+                    f"alternating grids 120x24x96 and 80x24x64, complex64 filter of degree {npl}; Sternheimer "
+                    f"solves of 48 right-hand sides ({npert} random displacement patterns, seed 1, x {nv} "
+                    "occupied states) on the converged FP64 120x24x96 channel, complex128, tol 1e-8",
+        "channels": nch, "columns": int(ncols.sum()),
+        "filter": {"ms_per_pass": round(filt_ms, 2), "GB/s": round(filt_gbs, 1),
+                   "frac_hbm": round(filt_gbs / HBM_PEAK_GBS, 4), "tuning": tuning},
+        "sternheimer": {"rhs": 48, "occupied": nv, "patterns": npert, "rhs_build_ms": round(rhs_ms, 1),
+                        "solve_ms": round(stern_ms, 1), "max_iterations": int(info.max_iterations),
+                        "ms_per_iteration": round(ms_it, 3), "converged": int(info.n_converged),
+                        "max_relres": float(info.max_relres),
+                        "dlambda_first_pattern": [round(float(v), 8) for v in dl[0][:4]]},
+        "one_graph": {"iterations_per_launch": k, "launches": launches, "ms": round(comb_ms, 1),
+                      "ms_per_launch": round(comb_ms / launches, 2),
+                      "sequential_ms": round(launches * filt_ms + stern_ms, 1),
+                      "converged": int(info2.n_converged), "max_iterations": int(info2.max_iterations)},
+    }
+
+
 def precision_check(states, npl, passes=3):
     """Mixed against FP64 on one Hamiltonian: both final blocks take the FP64
     path's converged Veff and `passes` more CheFSI passes; returns the
@@ -288,6 +415,9 @@ def main():
 
     head = "mixed" if "mixed" in results else paths[0]
     H = results[head]
+    batched = None
+    if not args.no_batched and "fp64" in states and args.workload == "cnt10_helical":
+        batched = batched_workload(dev, states["fp64"]["gs"], npl)
     err_fp64 = precision_check(states, npl) if ("fp64" in states and "mixed" in states) else None
     for st in states.values():
         st["gs"].close()
@@ -332,6 +462,7 @@ def main():
             if "fp64" in results and head != "fp64" else None),
         "error_vs_fp64": err_fp64,
         "poisson_1e-8": pois,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -88,14 +88,16 @@ def perturbation_projectors(proj, dchi):
 
 
 def sternheimer_rhs_batch(dev, system: CyclixSystem, kpt, psi, eigval, occ, bet, poisson, U, proj=None,
-                          poisson_tol=1e-8, poisson_maxit=3000):
+                          poisson_tol=1e-7, poisson_maxit=3000):
     """Right-hand sides of every displacement pattern in U (npert x n_atm x 3)
     for the occupied states of one channel, concatenated: pattern p owns
     columns [p nv, (p + 1) nv).  psi: (ns, N) device block of the channel's
     states (ascending eigenvalues), eigval and occ: host arrays.  Returns
     (B, lam, dlambda, dveff): the (npert nv, N) block, the per-column shifts
     lambda_n, the first-order eigenvalues (npert x nv) and the dVeff of each
-    pattern (device, FP64)."""
+    pattern (device, FP64).  dVeff is solved to poisson_tol: Jacobi-AAR
+    stalls near 1e-8 on these dipole-like charges at 120 x 24 x 96 (the
+    oracle's AAR reaches 1.2e-8 after 3000 iterations)."""
     import torch
 
     from .api import Hamiltonian
