@@ -5,20 +5,22 @@
 //
 // The 2.2 MB-per-vector problem is latency-bound, so an iteration is kept to
 // as few dependent launches as possible:
-//   Richardson step (two launches):
-//     1. theta_kernel: T = (-2 tau D_theta) x (helical cross term), plus one
-//        extra block that finalises the previous Richardson iteration: the
-//        fixed-order sum of its partials, relres and the convergence flag;
-//     2. step kernel: res1 = f - Lap x, res = res1 ./ diag, history columns,
-//        x_new = x + omega res (written speculatively into the other x
-//        buffer), per-block partials of ||res1||^2.  A step block takes 8
-//        z planes of one 64-point plane segment and stages x (with its z and
-//        r halo), T and the radial coefficients in LDS, so only the theta taps
-//        are read through the vector cache.
-//   Anderson step (four launches): theta (+ finalise of the previous
-//   iteration), step kernel with the DF'DF and DF'res partials, a one-block
-//   finalize that also computes gamma = pinv(DF'DF) DF'res on one wavefront,
-//   and x_new = x + beta res - (DX + beta DF) gamma.
+//   Richardson step (one launch): the step kernel forms T = (-2 tau D_theta) x
+//   (the helical cross term's theta half) for the planes it stages, then
+//   res1 = f - Lap x, res = res1 ./ diag, the history columns,
+//   x_new = x + omega res (written speculatively into another x buffer) and
+//   per-block partials of ||res1||^2; one extra block finalises the previous
+//   Richardson iteration (fixed-order sum of its partials, relres, the
+//   convergence flag).  A step block takes 8 z planes of one 64-point plane
+//   segment and stages x (with its z and r halo), T and the radial
+//   coefficients in LDS, so only the theta taps are read through the vector
+//   cache.
+//   Anderson step (three launches): the step kernel with the DF'DF and DF'res
+//   partials, a one-block finalize that also computes
+//   gamma = pinv(DF'DF) DF'res on one wavefront, and
+//   x_new = x + beta res - (DX + beta DF) gamma.
+// theta_kernel (the standalone T) serves the learned correction and carries
+// a finalise just before a poll.
 // Partials are stored column-major, part[d][block].  Step kernels of an
 // iteration after convergence store nothing, so the host enqueues iterations
 // in chunks and polls a device flag between chunks.  Every reduction has a
@@ -32,7 +34,7 @@ namespace cyx {
 
 struct AarState {
   int done;       // 1 converged, 2 hit max_iter
-  int iters;      // count at exit (src/aar.m:394-397 reports count-1 after the loop)
+  int iters;      // count at exit (src/aar.m:68-72 reports count-1 after the loop)
   int final_buf;  // which x buffer holds the returned iterate
   int pinv_eig;   // Anderson steps that took the eigen-decomposition pseudo-inverse
   double relres;  // relres of the returned iterate

@@ -318,7 +318,7 @@ int cyx_sternheimer_rhs(cyx_ham_t dham, int dtype, const void* psi, int64_t ldps
 
 /* ------------------------------------------------------------ Poisson */
 typedef struct cyx_aar_info {
-  int32_t iterations; /* count-1 at exit, as src/aar.m:394-397 reports */
+  int32_t iterations; /* count-1 at exit, as src/aar.m:68-72 reports */
   int32_t converged;  /* relres <= tol */
   double relres;      /* ||f - A x|| / ||f|| of the returned x */
   int32_t precond;    /* 0 Jacobi; 1 learned (cyx_poisson_set_preconditioner);

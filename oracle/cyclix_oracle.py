@@ -17,7 +17,11 @@ to the reference root ``Cyclix_MS_200220/``).  Conventions follow MATLAB:
   ``eta`` the helical/axial Bloch wavenumber (``src/initialization.m:1624-1641``).
 
 Parity pins (what fixes each function independently of its own formula) are
-listed in ``tests/test_oracle_*.py``; nothing here is "parity unpinned".
+listed in ``tests/test_oracle_*.py``.  One branch is "parity unpinned": the
+degenerate-pair entry 1 - (1 - occ_n) bet / 2 of ``coeff_pv_psi`` for
+partially occupied states (checked only by hand-computed entries of the same
+formula; with full occupation it is 1 and the perturbation-theory pin of the
+right-hand sides covers it).
 """
 from __future__ import annotations
 
