@@ -37,7 +37,7 @@ struct ColState {
   int active, half, iters, flag;  // flag: 0 ok, 1 not converged, 4 breakdown (rho or omega zero)
 };
 
-constexpr int SJB = 8;  // columns per thread in the finishing kernel
+constexpr int SJB = 16;  // columns per thread in the finishing kernel (psi_v is read once per SJB columns)
 
 // HX[j] = HX[j] - lam[j] X[j] + sum_v PV[v] C[v, j]   (C column-major nv x ncol);
 // SHIFT = false: HX[j] += sum_v PV[v] C[v, j] (X and lam unused)

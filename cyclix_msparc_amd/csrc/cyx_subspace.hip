@@ -218,16 +218,30 @@ __global__ void __launch_bounds__(256, 2) gram_mfma_kernel(const T* __restrict__
 
 // M (na x nb column-major) = fixed-order sum of the block partials; Hermitian
 // products keep the upper triangle and mirror it (conjugated) into the lower
+// A block takes GRE elements of one tile; its 256 threads are GRE elements x
+// GRL chunk lanes, lane l summing the chunks k = l mod GRL in order, and the
+// lanes' sums are added in lane order (a fixed order, so results are
+// reproducible), so that products with few tiles and many chunks still
+// spread over GS*GS/GRE blocks per tile.
+constexpr int GRE = 64, GRL = 4;
+
 template <class D>
-__global__ void gram_reduce_kernel(const D* __restrict__ part, int nchb, int herm, int nta, int na, int nb,
-                                   D* __restrict__ M) {
-  const int tp = blockIdx.x, e = threadIdx.x + blockDim.x * blockIdx.y;
-  if (e >= GS * GS) return;
+__global__ void __launch_bounds__(GRE * GRL) gram_reduce_kernel(const D* __restrict__ part, int nchb, int herm,
+                                                                int nta, int na, int nb, D* __restrict__ M) {
+  __shared__ D red[GRL][GRE];
+  const int tp = blockIdx.x;
+  const int el = threadIdx.x % GRE, lane = threadIdx.x / GRE;
+  const int e = blockIdx.y * GRE + el;
+  D s{};
+  for (int k = lane; k < nchb; k += GRL) s = add(s, part[((int64_t)tp * nchb + k) * (GS * GS) + e]);
+  red[lane][el] = s;
+  __syncthreads();
+  if (lane != 0) return;
+#pragma unroll
+  for (int l = 1; l < GRL; ++l) s = add(s, red[l][el]);
   int ta, tb;
   gram_tile_of(tp, herm != 0, nta, ta, tb);
   const int a = ta * GS + (e % GS), b = tb * GS + (e / GS);
-  D s{};
-  for (int k = 0; k < nchb; ++k) s = add(s, part[((int64_t)tp * nchb + k) * (GS * GS) + e]);
   if (a < na && b < nb) {
     if (!herm) {
       M[(int64_t)b * na + a] = s;
@@ -400,7 +414,9 @@ static GramShape gram_shape(int64_t N, int na, int nb, bool herm) {
   // whose 36 of 64 tiles leave the machine short of waves otherwise:
   // measured 4.3 against 4.46 ms per step for both products, with 4096 or
   // 32768 slower), k sub-ranges in whole k steps
-  int nchb = std::max(1, ceil_div(herm ? 16384 : 8192, 4 * g.ntp));
+  // (at most 512 chunks: a thin product, whose few tiles would otherwise be
+  // cut into thousands of chunks, then moves 16 MB of partials at most)
+  int nchb = std::max(1, std::min(512, ceil_div(herm ? 16384 : 8192, 4 * g.ntp)));
   const int kstep = 4 * gram_sp<T>();
   int64_t chunk = ceil_div(N, 4 * (int64_t)nchb);
   chunk = ceil_div(chunk, kstep) * kstep;
@@ -446,7 +462,7 @@ static int gram_impl(cyx_ham_s* h, const void* A, int64_t lda, int na, const voi
   hipLaunchKernelGGL(gram_mfma_kernel<T>, dim3(ntp, nchb), dim3(256), 0, st, (const T*)A, lda, na, (const T*)B, ldb,
                      nb, W, N, chunk, herm ? 1 : 0, nta, vec, (D*)part);
   CYX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / 256), dim3(256), 0, st, (const D*)part, nchb,
+  hipLaunchKernelGGL(gram_reduce_kernel<D>, dim3(ntp, GS * GS / GRE), dim3(GRE * GRL), 0, st, (const D*)part, nchb,
                      herm ? 1 : 0, nta, na, nb, (D*)M_out);
   CYX_HIP(hipGetLastError());
   return CYX_OK;
