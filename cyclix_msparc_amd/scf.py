@@ -37,7 +37,8 @@ def smearing_occ(ev, lam_f, bet, kind):
     """``src/occupations.m:11-15``: kind 0 Fermi-Dirac, 1 Gaussian."""
     ev = np.asarray(ev, dtype=float)
     if kind == 0:
-        return 1.0 / (1.0 + np.exp(bet * (ev - lam_f)))
+        from scipy.special import expit  # 1 / (1 + e^x) without overflow at large beta
+        return expit(-bet * (ev - lam_f))
     from scipy.special import erf
     return 0.5 * (1.0 - erf(bet * (ev - lam_f)))
 

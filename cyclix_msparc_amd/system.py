@@ -190,6 +190,38 @@ class Species:
         s = self.sigma_rho
         return self.Z / (2 * math.pi * s * s) ** 1.5 * np.exp(-np.asarray(d) ** 2 / (2 * s * s))
 
+    @property
+    def rc(self):
+        """Projector ball radius (S.Atm(ityp).rc)."""
+        return self.rc_nl
+
+    def projector_channels(self):
+        """(l, UdV radial form, gamma) per projector, in the reference's
+        (l, pp) order (``src/calculate_nloc_projector.m:211-236``)."""
+        return [(0, self.udv, self.gamma_nl)]
+
+
+def real_ylm(l, m, X, Y, Z):
+    """Real spherical harmonics of ``src/sphericalHarmonics.m`` (l <= 2) on
+    the vectors (X, Y, Z).  At the origin the l >= 1 functions are set to 0,
+    their limit along any ray times the vanishing radial factor of an l >= 1
+    projector (the reference's formula divides 0 by 0 there)."""
+    if l == 0:
+        return np.full(np.shape(X), Y00)
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rs = np.where(r > 0, r, 1.0)
+    if l == 1:
+        c = 0.488602511902920  # sqrt(3/(4 pi))
+        v = {-1: Y, 0: Z, 1: X}[m] / rs
+        return np.where(r > 0, c * v, 0.0)
+    if l == 2:
+        r2 = rs * rs
+        v = {-2: 1.092548430592079 * X * Y, -1: 1.092548430592079 * Y * Z,
+             0: 0.315391565252520 * (-X * X - Y * Y + 2 * Z * Z), 1: 1.092548430592079 * Z * X,
+             2: 0.546274215296040 * (X * X - Y * Y)}[m] / r2
+        return np.where(r > 0, v, 0.0)
+    raise ValueError("l > 2 is not supported")
+
 
 @dataclass
 class ProjImage:
@@ -231,7 +263,12 @@ class CyclixSystem:
     # ------------------------------------------------------------------
     def rb_xyz(self):
         """Per-direction support of the pseudocharge / projector boxes.
-        ``src/initialization.m:55-61,104-124`` (rb_y is an angle)."""
+        ``src/initialization.m:55-61,104-124`` (rb_y is an angle).  A species
+        that carries its own radii (found by the reference's search,
+        ``deck.find_rb``) supplies them as ``rb_xyz``."""
+        own = getattr(self.species, "rb_xyz", None)
+        if own is not None:
+            return tuple(own)
         g = self.grid
         R = self.atoms[:, 0].max()
         rb = self.species.rb
@@ -266,14 +303,20 @@ class CyclixSystem:
 
     def projectors(self):
         """KB projector tables.  ``src/calculate_nloc_projector.m:23-287``
-        (s channel only, psptyp 1 with one projector)."""
+        (psptyp 1: every (l, pp) projector of the species times the real
+        Y_lm, m = -l..l, evaluated on the node-minus-image vectors rotated
+        back by the image's angle, as ``ROT = S.RotM1^fac1 * S.RotM2^fac2``
+        does at ``:178-184``)."""
         g = self.grid
         sp = self.species
         rbx, rby, rbz = self.rb_xyz()
+        chans = sp.projector_channels()
+        gam = np.array([gm for (l, _, gm) in chans for _m in range(2 * l + 1)], dtype=float)
+        s_only = all(l == 0 for (l, _, _) in chans)
         out = []
         for J in range(self.n_atm):
             x0 = self.atoms[J]
-            A = ProjAtom(coords=x0.copy(), gamma=np.array([sp.gamma_nl]))
+            A = ProjAtom(coords=x0.copy(), gamma=gam.copy())
             for im in self._images(x0):
                 x0i, y0i, z0i = im
                 inx = ((x0i - g.xin) - (g.nr - 1) * g.dr + g.dr <= rbx) or abs(x0i - g.xin) <= rbx
@@ -293,12 +336,22 @@ class CyclixSystem:
                                          np.arange(k_s, k_e + 1), indexing="ij")
                 ii, jj, kk = (a.ravel(order="F") for a in (ii, jj, kk))
                 pts = np.stack([g.xin + (ii - 1) * g.dr, (jj - 1) * g.dt, (kk - 1) * g.dz], axis=1)
-                d = np.linalg.norm(to_cart(g, pts) - to_cart(g, im), axis=1)
-                ball = d <= sp.rc_nl
+                rel = to_cart(g, pts) - to_cart(g, im)
+                d = np.linalg.norm(rel, axis=1)
+                ball = d <= sp.rc
                 if not np.any(ball):
                     continue
                 rc_pos = ((kk[ball] - 1) * g.nr * g.nt + (jj[ball] - 1) * g.nr + ii[ball] - 1).astype(np.int32)
-                chi = (sp.udv(d[ball]) * Y00)[:, None]
+                if s_only:
+                    chi = np.stack([radial(d[ball]) * Y00 for (_, radial, _) in chans], axis=1)
+                else:
+                    P = rel[ball] @ self.image_rotation(J, im)  # rows R^T v: the reference's ROT v
+                    cols = []
+                    for (l, radial, _) in chans:
+                        u = radial(d[ball])
+                        for m in range(-l, l + 1):
+                            cols.append(u * real_ylm(l, m, P[:, 0], P[:, 1], P[:, 2]))
+                    chi = np.stack(cols, axis=1)
                 A.images.append(ProjImage(rc_pos=rc_pos, chi=chi, coords=np.array(im, dtype=float)))
             out.append(A)
         return out
