@@ -98,3 +98,20 @@ def test_real_spherical_harmonics_are_orthonormal():
     # scale-free, and 0 at the origin for l >= 1
     assert real_ylm(1, 1, np.array([2.0]), np.array([0.0]), np.array([0.0]))[0] == pytest.approx(math.sqrt(3 / (4 * math.pi)))
     assert real_ylm(1, 0, np.zeros(1), np.zeros(1), np.zeros(1))[0] == 0.0
+
+
+def test_oracle_scf_matches_the_reference_run():
+    """The oracle's whole SCF (``src/scf.m``) on the CNT_relax deck at its
+    first relaxation cell: free energy per atom against the reference run's
+    printed value (observed 5e-8 Ha/atom; about two minutes on 8 cores)."""
+    from oracle import cyclix_oracle as O
+    from tests.scf_common import oracle_inputs, psi0_for
+    D = load_deck(DECKS / "CNT_relax", "C", cell=RELAX_CELL)
+    s = D.system
+    g, atoms, pc = oracle_inputs(s)
+    o = D.options
+    res = O.scf(g, s.W, atoms, [(0.0, 0.0, 0.0)], [1.0], pc["b"], pc["rho_at"], pc["Eself"], pc["E_corr"],
+                s.nelectron, s.n_atm, [psi0_for(s)], s.npl, o["SCF_tol"], o["poisson_tol"], bet=o["bet"],
+                kind=o["elec_T_type"])
+    assert res["converged"]
+    assert abs(res["Eatom"] - REF["CNT_relax"]["free_energy_per_atom"]) <= 1e-6
