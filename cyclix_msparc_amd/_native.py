@@ -23,7 +23,7 @@ EXPORTS = (
     "cyx_ham_set_tuning",
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_chebyshev_filter_noalloc", "cyx_eigsolver_workspace_bytes",
     "cyx_eigsolver_pass",
-    "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
+    "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_weighted_gram", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
     "cyx_sternheimer_apply", "cyx_sternheimer_solve", "cyx_sternheimer_plan_create", "cyx_sternheimer_plan_iterate",
@@ -146,6 +146,7 @@ def load():
         "cyx_sternheimer_plan_destroy": (i32, [vp]),
         "cyx_sternheimer_rhs": (i32, [vp, i32, vp, i64, i32, pd, vp, i64, pd]),
         "cyx_filter_plan_attach_sternheimer": (i32, [vp, vp, i32]),
+        "cyx_weighted_gram": (i32, [vp, i32, vp, i64, i32, vp, i64, i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -362,6 +362,20 @@ class Hamiltonian:
                                                nat.dptr(cfc), C.c_void_p(B.data_ptr()), self.N, nat.dptr(dl)))
         return B, dl
 
+    def weighted_gram(self, A, B):
+        """A^H diag(W) B (FP64 accumulation) of two (n, N) device blocks of
+        one dtype: the reference's PsiW * X products.  Returns an (na, nb)
+        complex128 or float64 device tensor."""
+        _check_block(A, self.N)
+        _check_block(B, self.N)
+        if A.dtype != B.dtype:
+            raise ValueError("A and B must share a dtype")
+        out_dt = torch.complex128 if A.is_complex() else torch.float64
+        M = torch.empty((B.shape[0], A.shape[0]), dtype=out_dt, device=A.device)  # column-major na x nb
+        nat.check(self.lib.cyx_weighted_gram(self.h, _dtype_code(A), C.c_void_p(A.data_ptr()), self.N, A.shape[0],
+                                             C.c_void_p(B.data_ptr()), self.N, B.shape[0], C.c_void_p(M.data_ptr())))
+        return M.T
+
     def lanczos_bounds(self, v0, maxit=300, tol=1e-3):
         v0 = np.ascontiguousarray(np.asarray(v0, dtype=float))
         lo, hi = C.c_double(), C.c_double()

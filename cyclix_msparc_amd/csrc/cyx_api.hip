@@ -1042,6 +1042,17 @@ int cyx_ritz_residuals(cyx_ham_t h, int dtype, const void* psi, void* work, int 
   return CYX_OK;
 }
 
+int cyx_weighted_gram(cyx_ham_t h, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                      void* M) {
+  CYX_CHECK(h && A && B && M, "null argument");
+  int rc;
+  if ((rc = check_dtype(h, dtype)) != CYX_OK) return rc;
+  CYX_CHECK(na >= 1 && nb >= 1 && lda >= h->N() && ldb >= h->N(), "bad block shapes (na=%d nb=%d)", na, nb);
+  if ((rc = gram_rect(h, dtype, A, lda, na, B, ldb, nb, h->W, M, false)) != CYX_OK) return rc;
+  CYX_HIP(hipStreamSynchronize(h->ctx->stream));
+  return CYX_OK;
+}
+
 int cyx_lanczos_bounds(cyx_ham_t h, const double* v0, int maxit, double tol, double* lmin, double* lmax) {
   CYX_CHECK(h && v0 && lmin && lmax, "null argument");
   CYX_CHECK(maxit >= 5 && tol > 0, "bad Lanczos parameters");

@@ -469,7 +469,7 @@ class CyclixSystem:
         neg = -pos
         rho_at *= abs(neg / np.dot(W, rho_at))
         E_corr = 0.5 * np.sum((b_ref + b) * V_c * W) + Eself - Eself_ref
-        return dict(b=b, b_ref=b_ref, Eself=Eself, E_corr=E_corr, rho_at=rho_at,
+        return dict(b=b, b_ref=b_ref, Eself=Eself, E_corr=E_corr, rho_at=rho_at, V_c=V_c,
                     PosCharge=pos, NegCharge=neg)
 
     def pseudocharge_derivative(self, u):

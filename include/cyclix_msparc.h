@@ -183,6 +183,15 @@ int cyx_rayleigh_ritz(cyx_ham_t ham, int dtype, void* psi, void* work, int ns, d
 int cyx_ritz_residuals(cyx_ham_t ham, int dtype, const void* psi, void* work, int ns, const double* eigval,
                        double* resid);
 
+/* M = A^H diag(W) B over the grid (W the handle's quadrature weights):
+ * the PsiW * X products of the reference, e.g. the nonlocal force integrals
+ * transpose(conj(S.psi) .* S.W) * [Chi_full, DChi_x, DChi_y, DChi_z]
+ * (src/atomicForce.m:304-312).  A: device na columns (lda), B: device nb
+ * columns (ldb), both of dtype; M: device, na x nb column-major, FP64 or
+ * complex FP64 (accumulated in FP64).  Synchronises. */
+int cyx_weighted_gram(cyx_ham_t ham, int dtype, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                      void* M);
+
 /* Extreme Ritz values of H by Lanczos from start vector v0 (host, N doubles,
  * real part used); stands in for eigs(Hfun,N,1,'lr'/'sr') at
  * src/eigSolver.m:47-63 (tolerance TOL_LANCZOS, at most maxit steps). */
