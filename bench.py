@@ -53,8 +53,14 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per recurrence launch of the fused stencil at this workload, from
 # the rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
 # calibration) recorded in profiles/scf_step_cnt10_helical.md
-MEASURED_TRAFFIC = {("cnt10_helical", 16): 3.6670e+09, ("cnt10_helical", 8): 1.8445e+09}
+MEASURED_TRAFFIC = {("cnt10_helical", 16): 3.6668e+09, ("cnt10_helical", 8): 1.8164e+09}
 METRIC = "SCF step time @1e-5 Ha/atom, mixed+learned vs FP64 GPU; filter % HBM peak"
+_T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr when a phase of the benchmark ends."""
+    print(f"bench: {msg} ({time.perf_counter() - _T_START:.1f} s)", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -377,6 +383,7 @@ def main():
         precision = "fp64" if path == "fp64" else "fp32"
         results[path], states[path] = run_path(dev, sysm, args, precision,
                                                unet if (path == "mixed" and args.learned) else None)
+        progress(f"{path} path: {results[path]['ms_per_step']} ms per SCF step")
 
     # the workload's Poisson solve on its own: rho = sum of atom-centred
     # Gaussians, cold start, 1e-8 relative residual (src/poissonSolve.m)
@@ -397,6 +404,7 @@ def main():
                       "converged": bool(info8.converged), "relres": float(info8.relres)}
         if mode == "learned":
             unet.detach(poisson)
+        progress(f"standalone Poisson solve ({mode}): {pois[mode]['ms']} ms")
     if args.poisson_only:
         print(json.dumps({"workload": f"{args.workload}: grid {g.nr}x{g.nt}x{g.nz} (N={N}), rho = sum of "
                                       "atom-centred Gaussians, cold start, AAR m=7 p=6 to 1e-8",
@@ -418,7 +426,10 @@ def main():
     batched = None
     if not args.no_batched and "fp64" in states and args.workload == "cnt10_helical":
         batched = batched_workload(dev, states["fp64"]["gs"], npl)
+        progress("batched workload")
     err_fp64 = precision_check(states, npl) if ("fp64" in states and "mixed" in states) else None
+    if err_fp64 is not None:
+        progress("mixed against FP64")
     for st in states.values():
         st["gs"].close()
     poisson.close()
