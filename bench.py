@@ -80,6 +80,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-batched", action="store_true",
                    help="skip the batched workload (32 channels on two grids + 48 Sternheimer right-hand sides)")
+    p.add_argument("--no-dfpt", action="store_true",
+                   help="skip the DFPT self-consistency loop on the converged FP64 ground state")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
     p.add_argument("--poisson-only", action="store_true",
@@ -272,7 +274,7 @@ def batched_workload(dev, gs, npl):
     psi = gs.psi[0]
     t = time.perf_counter()
     B, lam, dl, _ = dfpt.sternheimer_rhs_batch(dev, gs.system, gs.kpts[0], psi, res.EigVal[:, 0], occ, gs.opt.bet,
-                                               gs.poisson, U, proj=projs[0])
+                                               gs.rho, U, proj=projs[0])
     torch.cuda.synchronize()
     rhs_ms = (time.perf_counter() - t) * 1e3
     B, lam = B[:48].contiguous(), lam[:48]
@@ -327,6 +329,30 @@ def batched_workload(dev, gs, npl):
                       "sequential_ms": round(launches * filt_ms + stern_ms, 1),
                       "converged": int(info2.n_converged), "max_iterations": int(info2.max_iterations)},
     }
+
+
+def dfpt_workload(gs):
+    """The DFPT self-consistency loop (src/phonon.m:195-366) for one random
+    displacement pattern (seed 1) on the converged FP64 ground state of the
+    120x24x96 channel, at the reference's tolerances (SCF_tol 1e-5, Poisson
+    1e-8, bicgstab 1e-8): iterations, time, and the Sternheimer and AAR
+    iteration counts of each DFPT iteration."""
+    import torch
+
+    from cyclix_msparc_amd import dfpt
+    u = dfpt.random_displacements(gs.system.n_atm, 1, seed=1)[0]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = dfpt.response_scf(gs, u)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    nv = dfpt.occupied_count(gs.result.occ[:, 0])
+    return {"workload": "self-consistent first-order response to one random displacement pattern (seed 1), "
+                        f"{nv} occupied states, complex128, SCF_tol 1e-5, Poisson 1e-8, bicgstab 1e-8",
+            "iterations": r.count, "converged": bool(r.converged), "ms": round(ms, 1),
+            "ms_per_iteration": round(ms / max(r.count, 1), 1), "errors": [float(f"{e:.3e}") for e in r.errors],
+            "bicgstab_iterations": [its[0] for its in r.sternheimer_iterations],
+            "aar_iterations": r.aar_iterations}
 
 
 def precision_check(states, npl, passes=3):
@@ -427,6 +453,10 @@ def main():
     if not args.no_batched and "fp64" in states and args.workload == "cnt10_helical":
         batched = batched_workload(dev, states["fp64"]["gs"], npl)
         progress("batched workload")
+    dfpt_rec = None
+    if not args.no_dfpt and "fp64" in states and args.workload == "cnt10_helical":
+        dfpt_rec = dfpt_workload(states["fp64"]["gs"])
+        progress(f"DFPT response: {dfpt_rec['iterations']} iterations, {dfpt_rec['ms']} ms")
     err_fp64 = precision_check(states, npl) if ("fp64" in states and "mixed" in states) else None
     if err_fp64 is not None:
         progress("mixed against FP64")
@@ -474,6 +504,7 @@ def main():
         "error_vs_fp64": err_fp64,
         "poisson_1e-8": pois,
         "batched": batched,
+        "dfpt": dfpt_rec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -24,6 +24,7 @@ EXPORTS = (
     "cyx_h_apply", "cyx_chebyshev_filter", "cyx_chebyshev_filter_noalloc", "cyx_eigsolver_workspace_bytes",
     "cyx_eigsolver_pass",
     "cyx_rayleigh_ritz", "cyx_ritz_residuals", "cyx_weighted_gram", "cyx_lanczos_bounds", "cyx_density_accumulate", "cyx_lda_pw", "cyx_energy_terms",
+    "cyx_density_response", "cyx_lda_pw_fxc",
     "cyx_poisson_create", "cyx_poisson_destroy", "cyx_aar",
     "cyx_poisson_solve", "cyx_scf_terms", "cyx_mixer_create", "cyx_mixer_destroy", "cyx_mixer_step",
     "cyx_sternheimer_apply", "cyx_sternheimer_solve", "cyx_sternheimer_plan_create", "cyx_sternheimer_plan_iterate",
@@ -109,6 +110,8 @@ def load():
         "cyx_lanczos_bounds": (i32, [vp, pd, i32, dbl, pd, pd]),
         "cyx_density_accumulate": (i32, [vp, i32, vp, i32, pd, dbl, vp]),
         "cyx_lda_pw": (i32, [vp, i64, vp, vp, vp, vp, dbl]),
+        "cyx_density_response": (i32, [vp, i32, vp, vp, i32, pd, pd, dbl, vp]),
+        "cyx_lda_pw_fxc": (i32, [vp, i64, vp, vp, vp, vp, vp, dbl]),
         "cyx_energy_terms": (i32, [vp, vp, vp, vp, vp, dbl, pd]),
         "cyx_poisson_create": (i32, [vp, C.POINTER(GridDesc), C.POINTER(vp)]),
         "cyx_poisson_destroy": (i32, [vp]),

@@ -278,6 +278,28 @@ class Hamiltonian:
                                                   nat.dptr(occ), float(weight), C.c_void_p(rho.data_ptr())))
         return rho
 
+    def density_response(self, drho, psi, dpsi, occ, dg=None, weight=1.0):
+        """drho += 2 w sum_s (dg_s |psi_s|^2 + 2 occ_s Re(conj(psi_s) dpsi_s))
+        (``src/phonon.m:1232-1240``, q = 0) over the first len(occ) states."""
+        occ = np.ascontiguousarray(np.asarray(occ, dtype=float))
+        ns = occ.size
+        _check_block(psi, self.N)
+        _check_block(dpsi, self.N)
+        if psi.dtype != dpsi.dtype or psi.shape[0] < ns or dpsi.shape[0] < ns:
+            raise ValueError("psi and dpsi must share a dtype and hold len(occ) states")
+        if drho.dtype != torch.float64 or drho.numel() != self.N or not drho.is_contiguous():
+            raise ValueError("drho must be a contiguous float64 vector of length N")
+        dgp = None
+        if dg is not None:
+            dg = np.ascontiguousarray(np.asarray(dg, dtype=float))
+            if dg.size != ns:
+                raise ValueError("dg must have len(occ) entries")
+            dgp = nat.dptr(dg)
+        nat.check(self.lib.cyx_density_response(self.h, _dtype_code(psi), C.c_void_p(psi.data_ptr()),
+                                                C.c_void_p(dpsi.data_ptr()), ns, nat.dptr(occ), dgp, float(weight),
+                                                C.c_void_p(drho.data_ptr())))
+        return drho
+
     def energy_terms(self, rho, vxc, phi, b, rhotol=1e-14):
         """(Exc, Exc_dc, Eelec_dc) grid sums (``src/evaluateTotalEnergy.m:27-52,72``)."""
         out = np.zeros(3)
@@ -537,6 +559,26 @@ def lda_pw(dev: Device, rho, phi=None, rhotol=1e-14):
                                  C.c_void_p(vxc.data_ptr()), C.c_void_p(veff.data_ptr()) if veff is not None else None,
                                  float(rhotol)))
     return vxc, veff
+
+
+def lda_pw_fxc(dev: Device, rho, drho=None, dphi=None, rhotol=1e-14, fxc=True):
+    """(fxc, dveff): the LDA-PW exchange-correlation kernel dVxc/drho at rho
+    (S.Dvxc_drho, ``src/phonon.m:215``) and, with drho and dphi,
+    DVeff = Dphi + fxc Drho (``src/phonon.m:303-306``); device FP64 vectors."""
+    if rho.dtype != torch.float64 or not rho.is_contiguous():
+        raise ValueError("rho must be a contiguous float64 vector")
+    for t in (drho, dphi):
+        if t is not None and (t.dtype != torch.float64 or t.numel() != rho.numel() or not t.is_contiguous()):
+            raise ValueError("drho and dphi must be contiguous float64 vectors of rho's length")
+    want_dveff = drho is not None and dphi is not None
+    if not (fxc or want_dveff):
+        raise ValueError("nothing to compute")
+    f = torch.empty_like(rho) if fxc else None
+    dveff = torch.empty_like(rho) if want_dveff else None
+    ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    nat.check(dev.lib.cyx_lda_pw_fxc(dev.h, rho.numel(), ptr(rho), ptr(drho), ptr(dphi), ptr(f), ptr(dveff),
+                                     float(rhotol)))
+    return f, dveff
 
 
 class Mixer:

@@ -1081,6 +1081,33 @@ int cyx_lda_pw(cyx_ctx_t ctx, int64_t N, const double* rho, const double* phi, d
   return lda_pw(ctx->stream, N, rho, phi, vxc, veff, rhotol);
 }
 
+int cyx_density_response(cyx_ham_t h, int dtype, const void* psi, const void* dpsi, int ns, const double* occ,
+                         const double* dg, double weight, double* drho) {
+  CYX_CHECK(h && psi && dpsi && occ && drho, "null argument");
+  int rc;
+  if ((rc = check_dtype(h, dtype)) != CYX_OK) return rc;
+  CYX_CHECK(ns >= 1, "bad state count");
+  if ((rc = ensure(h->sub, std::max(h->sub.bytes, 2 * (size_t)ns * sizeof(double)))) != CYX_OK) return rc;
+  double* coef = (double*)h->sub.p;
+  std::vector<double> hc(2 * (size_t)ns, 0.0);
+  for (int s = 0; s < ns; ++s) {
+    hc[s] = occ[s];
+    if (dg) hc[ns + s] = dg[s];
+  }
+  CYX_HIP(hipMemcpyAsync(coef, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, h->ctx->stream));
+  rc = density_response(h, dtype, psi, dpsi, ns, coef, 2.0 * weight, drho);
+  if (rc == CYX_OK) CYX_HIP(hipStreamSynchronize(h->ctx->stream));
+  return rc;
+}
+
+int cyx_lda_pw_fxc(cyx_ctx_t ctx, int64_t N, const double* rho, const double* drho, const double* dphi, double* fxc,
+                   double* dveff, double rhotol) {
+  CYX_CHECK(ctx && rho && N > 0, "null argument");
+  CYX_CHECK(fxc || dveff, "no output");
+  CYX_CHECK(dveff == nullptr || (drho && dphi), "dveff needs drho and dphi");
+  return lda_pw_fxc(ctx->stream, N, rho, drho, dphi, fxc, dveff, rhotol);
+}
+
 int cyx_energy_terms(cyx_ham_t h, const double* rho, const double* vxc, const double* phi, const double* b,
                      double rhotol, double out[3]) {
   CYX_CHECK(h && rho && vxc && phi && b && out, "null argument");

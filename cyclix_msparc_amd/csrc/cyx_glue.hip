@@ -26,6 +26,32 @@ __global__ void density_kernel(const T* __restrict__ psi, int64_t N, int64_t ld,
   rho[n] += c * acc;
 }
 
+// drho[n] += c * sum_s (dg[s] |psi_s[n]|^2 + 2 occ[s] Re(conj(psi_s[n]) dpsi_s[n]))
+// (src/phonon.m:1236-1238, q = 0); coef = {occ[0..ns), dg[0..ns)}
+template <class T>
+__global__ void density_response_kernel(const T* __restrict__ psi, const T* __restrict__ dpsi, int64_t N, int64_t ld,
+                                        int ns, const double* __restrict__ coef, double c,
+                                        double* __restrict__ drho) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double acc = 0;
+  for (int s = 0; s < ns; ++s) {
+    const T v = psi[(int64_t)s * ld + n];
+    const T d = dpsi[(int64_t)s * ld + n];
+    double a2, re;
+    if constexpr (traits<T>::is_complex) {
+      a2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      re = (double)v.x * (double)d.x + (double)v.y * (double)d.y;
+    } else {
+      a2 = (double)v * (double)v;
+      re = (double)v * (double)d;
+    }
+    acc = __builtin_fma(coef[ns + s], a2, acc);
+    acc = __builtin_fma(2.0 * coef[s], re, acc);
+  }
+  drho[n] += c * acc;
+}
+
 struct PW {
   static constexpr double p = 1.0, A = 0.031091, a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
   static constexpr double C2 = 0.73855876638202, C3 = 0.9847450218427;
@@ -49,6 +75,35 @@ __global__ void lda_pw_kernel(int64_t N, const double* __restrict__ rho_in, cons
   const double v = -PW::C3 * cbrt(rho) + vc;
   vxc[n] = v;
   if (veff) veff[n] = phi[n] + v;
+}
+
+// dVxc/drho of the potential above (the DFPT kernel S.Dvxc_drho,
+// src/phonon.m:215,303): PW92 differentiated in closed form through
+// rs = (3/(4 pi rho))^(1/3): dvc/drho = (2 ec'/3 - rs ec''/3)(-rs/(3 rho)),
+// dvx/drho = -C3 rho^(-2/3)/3; 0 where rho is clamped at rhotol.  Outputs
+// (each optional): fxc; dveff = dphi + fxc drho (src/phonon.m:303-306).
+__global__ void lda_pw_fxc_kernel(int64_t N, const double* __restrict__ rho_in, const double* __restrict__ drho,
+                                  const double* __restrict__ dphi, double* __restrict__ fxc,
+                                  double* __restrict__ dveff, double rhotol) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double r0 = rho_in[n];
+  double f = 0;
+  if (r0 >= rhotol) {
+    const double rs = pw_rs(r0);
+    const double srs = sqrt(rs);
+    const double G = 2 * PW::A * (PW::b1 * srs + PW::b2 * rs + PW::b3 * rs * srs + PW::b4 * rs * rs);
+    const double G1 = PW::A * (PW::b1 / srs + 2 * PW::b2 + 3 * PW::b3 * srs + 2 * (PW::p + 1) * PW::b4 * rs);
+    const double G2 = PW::A * (-0.5 * PW::b1 / (rs * srs) + 1.5 * PW::b3 / srs + 2 * PW::p * (PW::p + 1) * PW::b4);
+    const double D = G * G + G;
+    const double lg = log(1 + 1 / G);
+    const double ec1 = -2 * PW::A * PW::a1 * lg + 2 * PW::A * (1 + PW::a1 * rs) * G1 / D;
+    const double ec2 = 4 * PW::A * PW::a1 * G1 / D + 2 * PW::A * (1 + PW::a1 * rs) * (G2 * D - G1 * G1 * (2 * G + 1)) / (D * D);
+    const double cr = cbrt(r0);
+    f = (2 * ec1 / 3 - rs * ec2 / 3) * (-rs / (3 * r0)) - PW::C3 / (3 * cr * cr);
+  }
+  if (fxc) fxc[n] = f;
+  if (dveff) dveff[n] = dphi[n] + f * drho[n];
 }
 
 // per-block partial sums of: Exc (ec*rho*W - C2 rho^{4/3} W), Exc_dc (Vxc rho W), Eelec_dc (0.5 (b - rho) phi W)
@@ -110,6 +165,39 @@ int density_accumulate(cyx_ham_s* h, int dtype, const void* psi, int ns, const d
       return density_impl<cplx<float>>(h, psi, ns, occ_dev, c, rho);
   }
   return CYX_ERR_ARG;
+}
+
+template <class T>
+static int density_response_impl(cyx_ham_s* h, const void* psi, const void* dpsi, int ns, const double* coef_dev,
+                                 double c, double* drho) {
+  const int64_t N = h->N();
+  hipLaunchKernelGGL(density_response_kernel<T>, dim3(ceil_div(N, 256)), dim3(256), 0, h->ctx->stream, (const T*)psi,
+                     (const T*)dpsi, N, N, ns, coef_dev, c, drho);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
+}
+
+int density_response(cyx_ham_s* h, int dtype, const void* psi, const void* dpsi, int ns, const double* coef_dev,
+                     double c, double* drho) {
+  switch (dtype) {
+    case CYX_F64:
+      return density_response_impl<double>(h, psi, dpsi, ns, coef_dev, c, drho);
+    case CYX_F32:
+      return density_response_impl<float>(h, psi, dpsi, ns, coef_dev, c, drho);
+    case CYX_C128:
+      return density_response_impl<cplx<double>>(h, psi, dpsi, ns, coef_dev, c, drho);
+    case CYX_C64:
+      return density_response_impl<cplx<float>>(h, psi, dpsi, ns, coef_dev, c, drho);
+  }
+  return CYX_ERR_ARG;
+}
+
+int lda_pw_fxc(hipStream_t st, int64_t N, const double* rho, const double* drho, const double* dphi, double* fxc,
+               double* dveff, double rhotol) {
+  hipLaunchKernelGGL(lda_pw_fxc_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, rho, drho, dphi, fxc, dveff,
+                     rhotol);
+  CYX_HIP(hipGetLastError());
+  return CYX_OK;
 }
 
 int lda_pw(hipStream_t st, int64_t N, const double* rho, const double* phi, double* vxc, double* veff, double rhotol) {

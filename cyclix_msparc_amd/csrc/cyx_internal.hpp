@@ -265,6 +265,10 @@ int poisson_apply(cyx_poisson_s* ps, const double* x, double* y);
 // SCF glue
 int density_accumulate(cyx_ham_s* h, int dtype, const void* psi, int ns, const double* occ_dev, double c, double* rho);
 int lda_pw(hipStream_t st, int64_t N, const double* rho, const double* phi, double* vxc, double* veff, double rhotol);
+int density_response(cyx_ham_s* h, int dtype, const void* psi, const void* dpsi, int ns, const double* coef_dev,
+                     double c, double* drho);
+int lda_pw_fxc(hipStream_t st, int64_t N, const double* rho, const double* drho, const double* dphi, double* fxc,
+               double* dveff, double rhotol);
 int energy_terms(hipStream_t st, int64_t N, const double* rho, const double* vxc, const double* phi, const double* b,
                  const double* W, double rhotol, double* part_dev, int nblk, double out[3]);
 int veff_from_device(cyx_ham_s* h, const double* veff);

@@ -140,6 +140,17 @@ class PspSpecies:
         out[d > self.psp.r[-1]] = 0.0
         return out
 
+    def drho_iso_over_d(self, d):
+        """rho_iso'(d) / d from the spline; its limit rho_iso''(0) at d = 0."""
+        d = np.asarray(d, dtype=float)
+        out = np.zeros_like(d)
+        inside = d <= self.psp.r[-1]
+        small = inside & (d < 1e-8)
+        reg = inside & ~small
+        out[reg] = self._rho(d[reg], 1) / d[reg]
+        out[small] = self._rho(d[small], 2)
+        return out
+
     def projector_channels(self):
         return [(l, sp, g) for (l, sp, g) in self._proj]
 

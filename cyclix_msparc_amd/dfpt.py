@@ -7,16 +7,18 @@ channel (``src/phonon.m:1358-1444``, "Method-2", q = 0),
     dH = dVeff + sum_J gamma_J (chi_J dchi_J^H + dchi_J chi_J^H) W,
 
 and solves (H - lambda_n + psi_v psi_v^H W) x_n = rhs_n with bicgstab
-(``:1516-1521``).  dVeff at the first DFPT iteration is the potential of the
-pseudocharge change alone (S.Drho_dR = 0, ``:188,219``).
+(``:1516-1521``), inside the self-consistency loop over dVeff
+(``:195-366``, ``response_scf``).
 
 Set-up here (host, once per perturbation): the displacement patterns, the
-pseudocharge change db and the projector changes dchi (closed-form
-derivatives of the set-up tables, ``CyclixSystem.pseudocharge_derivative``
-and ``projector_derivatives``), the coefficient matrix coeff
-(``:112-158``).  On the device: dphi from the Poisson solve of db, the
-right-hand sides (``cyx_sternheimer_rhs``) and the batched solves
-(``SternheimerPlan``).
+pseudocharge change db, the atomic-density change (the loop's starting
+Drho) and the projector changes dchi (closed-form derivatives of the set-up
+tables, ``CyclixSystem.pseudocharge_derivative``,
+``atomic_density_derivative`` and ``projector_derivatives``), the
+coefficient matrix coeff (``:112-158``).  On the device: the right-hand
+sides (``cyx_sternheimer_rhs``), the batched solves (``SternheimerPlan``),
+the density response (``cyx_density_response``), Dphi (Jacobi-AAR), the
+exchange-correlation kernel (``cyx_lda_pw_fxc``) and the mixing.
 """
 from __future__ import annotations
 
@@ -87,20 +89,23 @@ def perturbation_projectors(proj, dchi):
     return out
 
 
-def sternheimer_rhs_batch(dev, system: CyclixSystem, kpt, psi, eigval, occ, bet, poisson, U, proj=None,
-                          poisson_tol=1e-7, poisson_maxit=3000):
+def sternheimer_rhs_batch(dev, system: CyclixSystem, kpt, psi, eigval, occ, bet, rho, U, proj=None,
+                          rhotol=1e-14):
     """Right-hand sides of every displacement pattern in U (npert x n_atm x 3)
-    for the occupied states of one channel, concatenated: pattern p owns
-    columns [p nv, (p + 1) nv).  psi: (ns, N) device block of the channel's
+    for the occupied states of one channel at the first iteration of the
+    DFPT loop, concatenated: pattern p owns columns [p nv, (p + 1) nv).
+    The first iteration's potential is the reference's starting point
+    (``src/phonon.m:195-219``): Drho = the atomic-density guess
+    (``CyclixSystem.atomic_density_derivative``), Dphi = 0, so
+    DVeff = fxc(rho) Drho with the LDA-PW kernel at the ground-state density
+    ``rho`` (device FP64).  psi: (ns, N) device block of the channel's
     states (ascending eigenvalues), eigval and occ: host arrays.  Returns
     (B, lam, dlambda, dveff): the (npert nv, N) block, the per-column shifts
-    lambda_n, the first-order eigenvalues (npert x nv) and the dVeff of each
-    pattern (device, FP64).  dVeff is solved to poisson_tol: Jacobi-AAR
-    stalls near 1e-8 on these dipole-like charges at 120 x 24 x 96 (the
-    oracle's AAR reaches 1.2e-8 after 3000 iterations)."""
+    lambda_n, the first-order eigenvalues (npert x nv) and the DVeff of each
+    pattern (device, FP64)."""
     import torch
 
-    from .api import Hamiltonian
+    from .api import Hamiltonian, lda_pw_fxc
 
     nv = occupied_count(occ)
     coeff = coeff_pv_psi(eigval, occ, bet, nv)
@@ -112,16 +117,160 @@ def sternheimer_rhs_batch(dev, system: CyclixSystem, kpt, psi, eigval, occ, bet,
     dveffs = []
     zero = torch.zeros(N, dtype=torch.float64, device=psi.device)
     for p, u in enumerate(U):
-        db = torch.from_numpy(system.pseudocharge_derivative(u)).to(psi.device)
-        dphi, info = poisson.solve(zero, db, tol=poisson_tol, max_iter=poisson_maxit)
-        if not info.converged:
-            raise RuntimeError(f"Poisson solve of the pseudocharge change did not reach {poisson_tol} "
-                               f"({info.iterations} iterations, relres {info.relres:.3e})")
-        dveffs.append(dphi)
-        dham = Hamiltonian(dev, system, kpt, dphi, perturbation_projectors(proj, system.projector_derivatives(u, proj)))
+        drho0 = torch.from_numpy(system.atomic_density_derivative(u)).to(psi.device)
+        _, dveff = lda_pw_fxc(dev, rho, drho0, zero, rhotol, fxc=False)
+        dveffs.append(dveff)
+        dham = Hamiltonian(dev, system, kpt, dveff, perturbation_projectors(proj, system.projector_derivatives(u, proj)))
         try:
             _, dl[p] = dham.sternheimer_rhs(psi, nv, coeff, out=B[p * nv:(p + 1) * nv])
         finally:
             dham.close()
     lam = np.tile(np.asarray(eigval, dtype=float)[:nv], npert)
     return B, lam, dl, dveffs
+
+
+def cartesian_patterns(n_atm):
+    """The reference's perturbations: one unit Cartesian displacement per
+    atom and direction, scf_natmc = 3 (J - 1) + dir (``src/phonon.m:235,1283-1286``)."""
+    U = np.zeros((3 * n_atm, n_atm, 3))
+    for k in range(3 * n_atm):
+        U[k, k // 3, k % 3] = 1.0
+    return U
+
+
+class ResponseResult:
+    """Self-consistent first-order response to one displacement pattern:
+    Drho, Dphi and the last output DVeff (device FP64), the first-order
+    eigenvalues (per channel, host) and Fermi level, the Sternheimer
+    solutions Dpsi (per channel, device blocks of the occupied states), the
+    iteration count, the SCF errors and the solver iteration counts."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def response_scf(gs, u, SCF_tol=1e-5, poisson_tol=None, lin_tol=1e-8, max_scf=100, min_scf=3, lin_maxit=3000,
+                 poisson_maxit=1000, dpsi0=None, drho0=None, proj=None, poll=8, verbose=False):
+    """The DFPT self-consistency loop of ``src/phonon.m:195-366`` (q = 0,
+    potential mixing) for one displacement pattern u (n_atm x 3) on a
+    converged FP64 ground state ``gs`` (``scf.GroundState`` after ``run``).
+
+    Start (``:195-256``): Drho = the atomic-density guess
+    (``CyclixSystem.atomic_density_derivative``), Dphi = 0,
+    DVeff = Dphi + fxc Drho, a cleared Periodic-Pulay history.  Each
+    iteration (``:259-359``), all on the device except the Ns-sized scalars:
+    per channel the Sternheimer right-hand sides of DVeff and the projector
+    changes (``cyx_sternheimer_rhs``) and their BiCGSTAB solves warm-started
+    from the previous solutions (a ``SternheimerPlan``; all ones at the
+    start, ``:224``); the first-order occupations (``occupation_pert``,
+    host); the density response (``cyx_density_response``); Dphi by
+    Jacobi-AAR on -4 pi (Drho + Db) from the previous Dphi; the output
+    DVeff = Dphi + fxc Drho (``cyx_lda_pw_fxc``); the error
+    ||DVeff_out - DVeff_in|| / ||DVeff_out||; Periodic-Pulay mixing unless
+    converged.  SCF_tol 1e-5, Poisson tolerance SCF_tol 1e-3 and the linear
+    solver tolerance 1e-8 are the reference's (``src/phonon.m:11-13``)."""
+    import torch
+
+    from .api import Hamiltonian, Mixer, SternheimerPlan, lda_pw_fxc
+    if gs.precision != "fp64":
+        raise ValueError("the response runs on an FP64 ground state")
+    if gs.result is None:
+        raise RuntimeError("the ground state has not been solved")
+    system = gs.system
+    dev = gs.dev
+    tdev = dev.torch_device
+    o = gs.opt
+    poisson_tol = SCF_tol * 1e-3 if poisson_tol is None else poisson_tol
+    u = np.asarray(u, dtype=float).reshape(system.n_atm, 3)
+    proj = system.projectors() if proj is None else proj
+    EigVal, occ = gs.result.EigVal, gs.result.occ
+    nch = len(gs.hams)
+    nvs = [occupied_count(occ[:, k]) for k in range(nch)]
+    coeffs = [coeff_pv_psi(EigVal[:, k], occ[:, k], o.bet, nvs[k]) for k in range(nch)]
+    pairs = perturbation_projectors(proj, system.projector_derivatives(u, proj))
+    db = torch.from_numpy(system.pseudocharge_derivative(u)).to(tdev)
+    if drho0 is None:
+        drho0 = system.atomic_density_derivative(u)
+    drho = torch.from_numpy(np.ascontiguousarray(drho0, dtype=float)).to(tdev)
+    dphi = torch.zeros(gs.N, dtype=torch.float64, device=tdev)
+    _, dveff_in = lda_pw_fxc(dev, gs.rho, drho, dphi, o.rhotol, fxc=False)
+    dhams = [Hamiltonian(dev, system, gs.kpts[k], dveff_in, pairs) for k in range(nch)]
+    mixer = Mixer(dev, dveff_in, o.MixingHistory, o.MixingParameter, o.MixingParameterSimple, o.PulayFrequency)
+    if dpsi0 is None:
+        dpsi = [torch.ones((nvs[k], gs.N), dtype=gs.psi[k].dtype, device=tdev) for k in range(nch)]
+    else:
+        dpsi = [d.clone().contiguous() for d in dpsi0]
+    B = [torch.empty((nvs[k], gs.N), dtype=gs.psi[k].dtype, device=tdev) for k in range(nch)]
+    lam = [np.asarray(EigVal[:nvs[k], k], dtype=float) for k in range(nch)]
+    err, count = 100.0, 1
+    errors, lin_its, aar_its = [], [], []
+    dlams = [None] * nch
+    dlf = 0.0
+    dveff_out = dveff_in
+    try:
+        while (err > SCF_tol and count <= max_scf) or count <= min_scf:
+            its_k = []
+            for k in range(nch):
+                dhams[k].set_veff(dveff_in)
+                _, dlams[k] = dhams[k].sternheimer_rhs(gs.psi[k], nvs[k], coeffs[k], out=B[k])
+                sp = SternheimerPlan(gs.hams[k], B[k], lam[k], gs.psi[k][:nvs[k]], X0=dpsi[k], tol=lin_tol)
+                try:
+                    act, info, its, rel = sp.solve(max_iter=lin_maxit, poll=poll)
+                    dpsi[k] = sp.X
+                finally:
+                    sp.close()
+                its_k.append(int(info.max_iterations))
+            lin_its.append(its_k)
+            dlf, dgs = _occupation_pert([occ[:nvs[k], k] for k in range(nch)], gs.wkpt, dlams, o.bet)
+            drho.zero_()
+            for k in range(nch):
+                gs.hams[k].density_response(drho, gs.psi[k], dpsi[k], occ[:nvs[k], k], dgs[k], gs.wkpt[k])
+            dphi, pinfo = gs.poisson.solve(drho, db, dphi, poisson_tol, poisson_maxit)
+            aar_its.append(int(pinfo.iterations))
+            _, dveff_out = lda_pw_fxc(dev, gs.rho, drho, dphi, o.rhotol, fxc=False)
+            _, dv2, v2 = gs.hams[0].scf_terms(dveff_out, dveff_in, drho)
+            err = math.sqrt(dv2 / v2)
+            errors.append(err)
+            if verbose:
+                print(f" DFPT SCF {count:3d}  err = {err:.4e}  bicgstab {its_k}  AAR {pinfo.iterations}", flush=True)
+            if err > SCF_tol or count < o.MINIT_SCF:
+                dveff_in = mixer.step(dveff_out, dveff_in, count)
+            else:
+                dveff_in = dveff_out
+            count += 1
+    finally:
+        mixer.close()
+        for h in dhams:
+            h.close()
+    return ResponseResult(drho=drho, dphi=dphi, dveff=dveff_out, dlambda=dlams, dlambda_f=dlf, dpsi=dpsi,
+                          count=count - 1, errors=errors, converged=err <= SCF_tol,
+                          sternheimer_iterations=lin_its, aar_iterations=aar_its)
+
+
+def _occupation_pert(occs, wkpt, dlams, bet):
+    """First-order Fermi level and occupations (``src/phonon.m:1254-1276``):
+    G = occ (1 - occ); DlambdaF = sum w G.dlambda / sum w G; Dg = -bet G
+    (dlambda - DlambdaF); both 0 when sum w G < 1e-4."""
+    num = den = 0.0
+    for oc, dl, wk in zip(occs, dlams, wkpt):
+        G = oc * (1 - oc)
+        num += wk * float(np.dot(G, dl))
+        den += wk * float(np.sum(G))
+    if den < 1e-4:
+        return 0.0, [np.zeros_like(oc) for oc in occs]
+    dlf = num / den
+    return dlf, [-bet * oc * (1 - oc) * (dl - dlf) for oc, dl in zip(occs, dlams)]
+
+
+def response_all(gs, patterns=None, verbose=False, **kw):
+    """The reference's loop over perturbations (``src/phonon.m:235``):
+    ``patterns`` (default: the 3 n_atm Cartesian unit displacements), each
+    solved by ``response_scf`` with the Sternheimer solutions of the previous
+    pattern as the starting guess, as S.Dpsi_dR_kq carries them there."""
+    patterns = cartesian_patterns(gs.system.n_atm) if patterns is None else patterns
+    out, dpsi = [], None
+    for p, u in enumerate(patterns):
+        r = response_scf(gs, u, dpsi0=dpsi, verbose=verbose, **kw)
+        dpsi = r.dpsi
+        out.append(r)
+    return out

@@ -190,6 +190,10 @@ class Species:
         s = self.sigma_rho
         return self.Z / (2 * math.pi * s * s) ** 1.5 * np.exp(-np.asarray(d) ** 2 / (2 * s * s))
 
+    def drho_iso_over_d(self, d):
+        """rho_iso'(d) / d of the Gaussian atomic density."""
+        return -self.rho_iso(d) / self.sigma_rho ** 2
+
     @property
     def rc(self):
         """Projector ball radius (S.Atm(ityp).rc)."""
@@ -467,10 +471,11 @@ class CyclixSystem:
         Eself_ref *= -1 / (4 * math.pi)
         pos = abs(np.dot(W, b))
         neg = -pos
-        rho_at *= abs(neg / np.dot(W, rho_at))
+        rho_scale = abs(neg / np.dot(W, rho_at))
+        rho_at *= rho_scale
         E_corr = 0.5 * np.sum((b_ref + b) * V_c * W) + Eself - Eself_ref
         return dict(b=b, b_ref=b_ref, Eself=Eself, E_corr=E_corr, rho_at=rho_at, V_c=V_c,
-                    PosCharge=pos, NegCharge=neg)
+                    PosCharge=pos, NegCharge=neg, rho_scale=rho_scale)
 
     def pseudocharge_derivative(self, u):
         """First-order change of the pseudocharge b under the Cartesian atom
@@ -511,6 +516,44 @@ class CyclixSystem:
                 rows = ((KKb - 1) * g.nr * g.nt + (JJb - 1) * g.nr + IIb - 1).ravel()
                 np.add.at(db, rows, dbJ.ravel())
         return db * (-1 / (4 * math.pi))
+
+    def atomic_density_derivative(self, u, scale=None):
+        """First-order change of the superposed atomic density rho_at under
+        the Cartesian displacements u (n_atm x 3): the initial density
+        response S.Drho_dR of the DFPT loop (``electrostatics_local``,
+        ``src/phonon.m:2402-2408``: minus the field-point gradient of each
+        image's density on its pseudocharge box, rotated into the atom's
+        frame).  ``scale`` is the normalisation that ``pseudocharge()``
+        applies to rho_at (default: the one at the current geometry)."""
+        g = self.grid
+        sp = self.species
+        u = np.asarray(u, dtype=float).reshape(self.n_atm, 3)
+        if scale is None:
+            scale = self.pseudocharge()["rho_scale"]
+        drho = np.zeros(g.N)
+        rbx, rby, rbz = self.rb_xyz()
+        for J in range(self.n_atm):
+            for im in self._images(self.atoms[J]):
+                x0i, y0i, z0i = im
+                i_s = max(math.ceil(((x0i - g.xin) - rbx) / g.dr) + 1, 1)
+                i_e = min(math.floor(((x0i - g.xin) + rbx) / g.dr) + 1, g.nr)
+                j_s = max(math.ceil((y0i - rby) / g.dt) + 1, 1)
+                j_e = min(math.floor((y0i + rby) / g.dt) + 1, g.nt)
+                k_s = max(math.ceil((z0i - rbz) / g.dz) + 1, 1)
+                k_e = min(math.floor((z0i + rbz) / g.dz) + 1, g.nz)
+                if i_e < i_s or j_e < j_s or k_e < k_s:
+                    continue
+                II, JJ, KK = np.meshgrid(np.arange(i_s, i_e + 1), np.arange(j_s, j_e + 1), np.arange(k_s, k_e + 1),
+                                         indexing="ij")
+                pts = np.stack([g.xin + (II.ravel() - 1) * g.dr, (JJ.ravel() - 1) * g.dt, (KK.ravel() - 1) * g.dz],
+                               axis=1)
+                diff = to_cart(g, pts) - to_cart(g, im)
+                dd = np.linalg.norm(diff, axis=1)
+                v = self.image_rotation(J, im) @ u[J]
+                # d rho(|x - X|)/dX . v = -rho'(d)/d (x - X) . v
+                rows = ((KK - 1) * g.nr * g.nt + (JJ - 1) * g.nr + II - 1).ravel()
+                np.add.at(drho, rows, -sp.drho_iso_over_d(dd) * (diff @ v))
+        return drho * scale
 
 
 # --------------------------------------------------------------------------

@@ -240,6 +240,21 @@ int cyx_density_accumulate(cyx_ham_t ham, int dtype, const void* psi, int ns, co
  * veff = phi + vxc (src/scf.m:39).  All device, length N. */
 int cyx_lda_pw(cyx_ctx_t ctx, int64_t N, const double* rho, const double* phi, double* vxc, double* veff,
                double rhotol);
+/* DFPT density response at q = 0 (electronDensity_pert,
+ * src/phonon.m:1232-1240): drho += 2 weight sum_s (dg[s] |psi_s|^2
+ * + 2 occ[s] Re(conj(psi_s) dpsi_s)) over ns states; psi and dpsi are
+ * (ns, N) device blocks of one dtype, occ and dg host arrays of ns (dg may
+ * be null: 0, an insulator); drho an FP64 device vector of N. */
+int cyx_density_response(cyx_ham_t ham, int dtype, const void* psi, const void* dpsi, int ns, const double* occ,
+                         const double* dg, double weight, double* drho);
+/* DFPT exchange-correlation kernel S.Dvxc_drho of the LDA Perdew-Wang
+ * potential (src/phonon.m:215,303; the reference has it only for LDA_PZ via
+ * libxc, src/exchangeCorrelationPotential.m:125): the closed-form derivative
+ * of cyx_lda_pw's potential, 0 where rho < rhotol.  fxc (optional) receives
+ * the kernel; dveff (optional) = dphi + fxc .* drho (src/phonon.m:303-306).
+ * All device, length N. */
+int cyx_lda_pw_fxc(cyx_ctx_t ctx, int64_t N, const double* rho, const double* drho, const double* dphi, double* fxc,
+                   double* dveff, double rhotol);
 /* out = {Exc, Exc_dc, Eelec_dc}: the grid sums of evaluateTotalEnergy
  * (src/evaluateTotalEnergy.m:27-52,72) with the handle's weights S.W. */
 int cyx_energy_terms(cyx_ham_t ham, const double* rho, const double* vxc, const double* phi, const double* b,

@@ -615,6 +615,36 @@ def lda_pw_vxc(rho, rhotol=1e-14):
     return -C3 * rho ** (1 / 3) + vc
 
 
+def lda_pw_fxc(rho, rhotol=1e-14):
+    """dVxc/drho of the LDA (Perdew-Wang) potential ``lda_pw_vxc``: the kernel
+    S.Dvxc_drho that the DFPT loop multiplies the density response by
+    (``src/phonon.m:215,303``).  The reference sets it only for LDA_PZ through
+    libxc (``src/exchangeCorrelationPotential.m:125``); this is the closed-form
+    derivative of the PW92 expressions of ``src/exchangeCorrelationPotential.m:82-106``
+    (J. P. Perdew and Y. Wang, Phys. Rev. B 45, 13244 (1992)) with respect to
+    rho: with rs = (3/(4 pi rho))^(1/3), drs/drho = -rs/(3 rho),
+    vc = ec - rs ec'/3 and dvc/drho = (2 ec'/3 - rs ec''/3) drs/drho; the
+    exchange part -C3 rho^(1/3) gives -C3 rho^(-2/3)/3.  Below rhotol the
+    potential is constant (rho is clamped there), so the kernel is 0."""
+    rho = np.asarray(rho, dtype=float)
+    clamp = rho < rhotol
+    r = np.maximum(rho, rhotol)
+    p, A, a1 = _PW["p"], _PW["A"], _PW["alpha1"]
+    b1, b2, b3, b4 = _PW["beta1"], _PW["beta2"], _PW["beta3"], _PW["beta4"]
+    C3 = 0.9847450218427
+    rs = (0.75 / (np.pi * r)) ** (1 / 3)
+    G = 2 * A * (b1 * rs ** 0.5 + b2 * rs + b3 * rs ** 1.5 + b4 * rs ** (p + 1))
+    G1 = A * (b1 * rs ** -0.5 + 2 * b2 + 3 * b3 * rs ** 0.5 + 2 * (p + 1) * b4 * rs ** p)
+    G2 = A * (-0.5 * b1 * rs ** -1.5 + 1.5 * b3 * rs ** -0.5 + 2 * p * (p + 1) * b4 * rs ** (p - 1))
+    D = G * G + G
+    L = np.log(1 + 1 / G)
+    ec1 = -2 * A * a1 * L + 2 * A * (1 + a1 * rs) * G1 / D
+    ec2 = 4 * A * a1 * G1 / D + 2 * A * (1 + a1 * rs) * (G2 * D - G1 * G1 * (2 * G + 1)) / (D * D)
+    fc = (2 * ec1 / 3 - rs * ec2 / 3) * (-rs / (3 * r))
+    fx = -C3 / 3 * r ** (-2 / 3)
+    return np.where(clamp, 0.0, fx + fc)
+
+
 def lda_pw_exc(rho, W, rhotol=1e-14):
     """Exchange-correlation energy.  ``src/evaluateTotalEnergy.m:27-47``."""
     rho = np.maximum(rho, rhotol)
@@ -783,6 +813,104 @@ def sternheimer_rhs(W, kpt, dveff, atoms, datoms, psi, coeff):
     psi_DH_psi = psi_DVeff_psi + psi_DVnl_psi
     rhs = psi @ (psi_DH_psi * coeff) + rhs_nl - DVeff_psi
     return rhs, np.real(np.diag(psi_DH_psi))
+
+
+def occupation_pert(occs, wkpt, dlams, bet):
+    """First-order Fermi level and occupations at q = 0,
+    ``src/phonon.m:1254-1276``: with G = occ (1 - occ) over the occupied
+    states of each channel, DlambdaF = sum_k w_k G.dlambda / sum_k w_k sum G
+    and Dg = -bet G (dlambda - DlambdaF); both 0 when the denominator is
+    below 1e-4 (an insulator).  occs, dlams: one array per channel (the
+    occupied states).  Returns (DlambdaF, [Dg per channel])."""
+    num = den = 0.0
+    for oc, dl, wk in zip(occs, dlams, wkpt):
+        G = oc * (1 - oc)
+        num += wk * np.dot(G, dl)
+        den += wk * np.sum(G)
+    if den < 1e-4:
+        return 0.0, [np.zeros_like(oc) for oc in occs]
+    dlf = num / den
+    return dlf, [-bet * oc * (1 - oc) * (dl - dlf) for oc, dl in zip(occs, dlams)]
+
+
+def electron_density_pert(psis, dpsis, occs, dgs, wkpt):
+    """First-order density at q = 0, ``src/phonon.m:1232-1240``:
+    Drho = sum_k 2 w_k (sum_n Dg_n |psi_n|^2 + 2 Re sum_n occ_n Dpsi_n conj(psi_n))
+    over the occupied states (psis[k], dpsis[k]: N x nv_k)."""
+    drho = 0.0
+    for psi, dpsi, oc, dg, wk in zip(psis, dpsis, occs, dgs, wkpt):
+        drho = drho + 2 * wk * (np.sum((psi * psi.conj()).real * dg[None, :], axis=1)
+                                + 2 * np.real(np.sum(dpsi * np.conj(psi) * oc[None, :], axis=1)))
+    return np.real(drho)
+
+
+def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db, drho0, bet,
+             SCF_tol=1e-5, poisson_tol=1e-8, lin_tol=1e-8, max_scf=100, min_scf=3, MINIT_SCF=3,
+             hist=7, beta=0.3, omega=None, pfreq=1, lin_maxit=3000, poisson_maxit=1000, rhotol=1e-14,
+             poisson=None, dpsi0=None, linsolve=None):
+    """Self-consistent first-order response to one atomic-displacement
+    pattern at q = 0 with potential mixing, ``src/phonon.m:195-366``:
+
+    * start: Drho = drho0 (the atomic-density guess of ``electrostatics_local``,
+      ``:2405-2408``), Dphi = 0 (``:207``), DVxc = fxc Drho, DVeff = Dphi + DVxc
+      (``:212-219``); the mixing history cleared, its x_{k-1} = DVeff (``:236-256``);
+    * each iteration (``:259-359``): per channel the Sternheimer right-hand
+      sides of DVeff and the projector changes and their bicgstab solves,
+      warm-started from the previous solutions (``sternheimer`` ``:1279-1521``;
+      ones at the very start, ``:224``); ``occupation_pert``;
+      ``electronDensity_pert``; Dphi from AAR on f = -4 pi (Drho + Db) from the
+      previous Dphi (``aar_phonon`` with the Laplacian, ``:286-293``);
+      DVeff_out = Dphi + fxc Drho; err = ||DVeff_out - DVeff_in|| / ||DVeff_out||;
+      Periodic-Pulay mixing unless converged (``:326-334``).
+
+    hams: per channel the oracle Hamiltonian at the ground state (its veff);
+    atoms, datoms: projector tables and their changes per channel (image
+    phases come from each channel's kpt); psis: occupied states per channel
+    (N x nv); rho: ground-state density (for fxc).  ``poisson(drho, db, dphi)``
+    overrides the Jacobi-AAR solve and ``linsolve(k, lam, psi, rhs, x0)`` the
+    Sternheimer bicgstab of channel k.  Returns a dict with Drho, Dphi, Dveff
+    (the last output potential), Dlambda and Dpsi per channel, DlambdaF,
+    count and the error history."""
+    fxc = lda_pw_fxc(rho, rhotol)
+    nch = len(hams)
+    coeffs = [coeff_pv_psi(ev, oc, bet, psi.shape[1]) for ev, oc, psi in zip(eigvals, occs, psis)]
+    if poisson is None:
+        def poisson(drho, dbb, guess):
+            x, _, _ = poisson_solve(g, drho, dbb, poisson_tol, guess, poisson_maxit, "jacobi")
+            return x
+    drho = np.array(drho0, dtype=float, copy=True)
+    dphi = np.zeros(g.N)
+    dveff_in = dphi + fxc * drho
+    mix = PeriodicPulay(g.N, hist, beta, omega, pfreq)
+    dpsis = ([np.ones(p.shape, dtype=complex) for p in psis] if dpsi0 is None
+             else [np.array(d, copy=True) for d in dpsi0])
+    err, count = 100.0, 1
+    errs = []
+    dlams = [None] * nch
+    dlf = 0.0
+    dveff_out = dveff_in
+    while (err > SCF_tol and count <= max_scf) or count <= min_scf:
+        for k in range(nch):
+            psi = psis[k].astype(complex)
+            rhs, dlams[k] = sternheimer_rhs(W, hams[k].kpt, dveff_in, atoms[k], datoms[k], psi, coeffs[k])
+            lam = eigvals[k][:psi.shape[1]]
+            if linsolve is None:
+                dpsis[k], _, _, _ = sternheimer_solve(hams[k], lam, psi, rhs, dpsis[k], lin_tol, lin_maxit, True)
+            else:
+                dpsis[k] = linsolve(k, lam, psi, rhs, dpsis[k])
+        dlf, dgs = occupation_pert([oc[:p.shape[1]] for oc, p in zip(occs, psis)], wkpt, dlams, bet)
+        drho = electron_density_pert(psis, dpsis, [oc[:p.shape[1]] for oc, p in zip(occs, psis)], dgs, wkpt)
+        dphi = poisson(drho, db, dphi)
+        dveff_out = dphi + fxc * drho
+        err = np.linalg.norm(dveff_out - dveff_in) / np.linalg.norm(dveff_out)
+        errs.append(err)
+        if err > SCF_tol or count < MINIT_SCF:
+            dveff_in = mix(dveff_out, dveff_in, count)
+        else:
+            dveff_in = dveff_out
+        count += 1
+    return dict(Drho=drho, Dphi=dphi, Dveff=dveff_out, Dlambda=dlams, Dpsi=dpsis, DlambdaF=dlf,
+                count=count - 1, errors=errs, converged=err <= SCF_tol)
 
 
 # --------------------------------------------------------------------------

@@ -165,7 +165,7 @@ def test_dfpt_48_rhs_on_the_production_channel(device):
     kpt = gs.kpts[0]
     psi = gs.psi[0]
     B, lam, dl, dveffs = dfpt.sternheimer_rhs_batch(device, sysm, kpt, psi, res.EigVal[:, 0], occ, gs.opt.bet,
-                                                    gs.poisson, U, proj=proj)
+                                                    gs.rho, U, proj=proj)
     assert B.shape == (npert * nv, g.N)
     coeff = dfpt.coeff_pv_psi(res.EigVal[:, 0], occ, gs.opt.bet, nv)
     P = psi[:nv].cpu().numpy().T
