@@ -2,6 +2,7 @@
 // step, the KB gather / scatter kernels around it, and the batched filter
 // plans (many channels per launch, captured in a HIP graph).  The stencil
 // kernels themselves are in cyx_stencil_kernel.hpp.
+#include <cstring>
 #include <mutex>
 #include <type_traits>
 
@@ -45,21 +46,18 @@ __global__ void __launch_bounds__(256) kb_gather_kernel(const T* __restrict__ Y,
 constexpr int KSC = 8;
 
 template <class T, class TS = T>
-__global__ void __launch_bounds__(256) kb_scatter_kernel(TS* __restrict__ Y, int64_t ld, int ncol,
-                                                         const int* __restrict__ kb_nodes, int nkbn,
-                                                         const uint32_t* __restrict__ s_pk,
-                                                         const T* __restrict__ s_cph, const int* __restrict__ s_pj,
-                                                         const T* __restrict__ C, int nproj,
-                                                         typename traits<T>::real alpha) {
-  extern __shared__ __align__(16) unsigned char ksc_raw[];
-  T* Cs = reinterpret_cast<T*>(ksc_raw);  // [KSC][nproj]
-  const int s0 = blockIdx.y * KSC;
+__device__ __forceinline__ void kb_scatter_body(TS* __restrict__ Y, int64_t ld, int ncol,
+                                                const int* __restrict__ kb_nodes, int nkbn,
+                                                const uint32_t* __restrict__ s_pk, const T* __restrict__ s_cph,
+                                                const int* __restrict__ s_pj, const T* __restrict__ C, int nproj,
+                                                typename traits<T>::real alpha, int bx, int by, T* Cs) {
+  const int s0 = by * KSC;
   for (int e = threadIdx.x; e < KSC * nproj; e += 256) {
     const int c = e / nproj, p = e % nproj;
     Cs[e] = s0 + c < ncol ? C[(int64_t)(s0 + c) * nproj + p] : zero<T>();
   }
   __syncthreads();
-  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int q = bx * 256 + threadIdx.x;
   if (q >= nkbn) return;
   const int node = kb_nodes[q];
   // the node's values are loaded first, so that their latency overlaps the
@@ -83,6 +81,18 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(TS* __restrict__ Y, int
     if (s0 + c < ncol) Y[(int64_t)(s0 + c) * ld + node] = to_store<TS>(rfma(alpha, acc[c], yv[c]));
 }
 
+template <class T, class TS = T>
+__global__ void __launch_bounds__(256) kb_scatter_kernel(TS* __restrict__ Y, int64_t ld, int ncol,
+                                                         const int* __restrict__ kb_nodes, int nkbn,
+                                                         const uint32_t* __restrict__ s_pk,
+                                                         const T* __restrict__ s_cph, const int* __restrict__ s_pj,
+                                                         const T* __restrict__ C, int nproj,
+                                                         typename traits<T>::real alpha) {
+  extern __shared__ __align__(16) unsigned char ksc_raw[];
+  kb_scatter_body<T, TS>(Y, ld, ncol, kb_nodes, nkbn, s_pk, s_cph, s_pj, C, nproj, alpha, blockIdx.x, blockIdx.y,
+                         reinterpret_cast<T*>(ksc_raw));  // Cs: [KSC][nproj]
+}
+
 // Two-stage gather for the single-channel step.  The node range is cut into
 // chunks shared by all projectors; block (pair, column group) sums
 // g[e] * Y_s[node[e]] over one projector's entries in one chunk for GCOL
@@ -96,19 +106,17 @@ __global__ void __launch_bounds__(256) kb_scatter_kernel(TS* __restrict__ Y, int
 constexpr int GCOL = 8;
 
 template <class T, class TS = T>
-__global__ void __launch_bounds__(256) kb_gather_part_kernel(const TS* __restrict__ Y, int64_t ld, int ncol,
-                                                             const int* __restrict__ blk_pair,
-                                                             const int* __restrict__ pr_lo,
-                                                             const int* __restrict__ pr_hi,
-                                                             const int* __restrict__ g_node,
-                                                             const T* __restrict__ g_coef, int npair,
-                                                             typename acc_of<T>::type* __restrict__ part) {
+__device__ __forceinline__ void kb_gather_part_body(const TS* __restrict__ Y, int64_t ld, int ncol,
+                                                    const int* __restrict__ blk_pair, const int* __restrict__ pr_lo,
+                                                    const int* __restrict__ pr_hi, const int* __restrict__ g_node,
+                                                    const T* __restrict__ g_coef, int npair,
+                                                    typename acc_of<T>::type* __restrict__ part, int bx, int by,
+                                                    double* red) {
   constexpr bool CX = traits<T>::is_complex;
   constexpr int K = CX ? 2 * GCOL : GCOL;
-  __shared__ double red[(256 / 64) * K];
-  const int q = blk_pair[blockIdx.x];
+  const int q = blk_pair[bx];
   if (q < 0) return;  // padding of the XCD-ordered launch (whole block)
-  const int s0 = blockIdx.y * GCOL;
+  const int s0 = by * GCOL;
   const int nc = min(GCOL, ncol - s0);
   const int c0 = pr_lo[q], c1 = pr_hi[q];
   T acc[GCOL];
@@ -151,16 +159,29 @@ __global__ void __launch_bounds__(256) kb_gather_part_kernel(const TS* __restric
   }
 }
 
+template <class T, class TS = T>
+__global__ void __launch_bounds__(256) kb_gather_part_kernel(const TS* __restrict__ Y, int64_t ld, int ncol,
+                                                             const int* __restrict__ blk_pair,
+                                                             const int* __restrict__ pr_lo,
+                                                             const int* __restrict__ pr_hi,
+                                                             const int* __restrict__ g_node,
+                                                             const T* __restrict__ g_coef, int npair,
+                                                             typename acc_of<T>::type* __restrict__ part) {
+  constexpr int K = traits<T>::is_complex ? 2 * GCOL : GCOL;
+  __shared__ double red[(256 / 64) * K];
+  kb_gather_part_body<T, TS>(Y, ld, ncol, blk_pair, pr_lo, pr_hi, g_node, g_coef, npair, part, blockIdx.x, blockIdx.y,
+                             red);
+}
+
 // one wavefront per (column, projector): lane k sums pairs k, k + 64, ...
 // of the projector (contiguous, chunk order), then a fixed butterfly
 template <class T>
-__global__ void __launch_bounds__(256) kb_gather_reduce_kernel(const typename acc_of<T>::type* __restrict__ part,
-                                                               int ncol, int nproj, int npair,
-                                                               const int* __restrict__ pp_ptr,
-                                                               const typename traits<T>::real* __restrict__ gamma,
-                                                               T* __restrict__ C) {
+__device__ __forceinline__ void kb_gather_reduce_body(const typename acc_of<T>::type* __restrict__ part, int ncol,
+                                                      int nproj, int npair, const int* __restrict__ pp_ptr,
+                                                      const typename traits<T>::real* __restrict__ gamma,
+                                                      T* __restrict__ C, int bx) {
   using D = typename acc_of<T>::type;
-  const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);  // s * nproj + p
+  const int idx = bx * 4 + (threadIdx.x >> 6);  // s * nproj + p
   const int lane = threadIdx.x & 63;
   if (idx >= ncol * nproj) return;  // whole wavefronts
   const int s = idx / nproj, p = idx - s * nproj;
@@ -177,39 +198,81 @@ __global__ void __launch_bounds__(256) kb_gather_reduce_kernel(const typename ac
   }
 }
 
-// Batched gather: entry c owns blocks [blk0, blk0 + nproj * ncol)
 template <class T>
-struct GatherEntry {
+__global__ void __launch_bounds__(256) kb_gather_reduce_kernel(const typename acc_of<T>::type* __restrict__ part,
+                                                               int ncol, int nproj, int npair,
+                                                               const int* __restrict__ pp_ptr,
+                                                               const typename traits<T>::real* __restrict__ gamma,
+                                                               T* __restrict__ C) {
+  kb_gather_reduce_body<T>(part, ncol, nproj, npair, pp_ptr, gamma, C, blockIdx.x);
+}
+
+// Batched filter plans run the same three KB passes for all channels in one
+// launch each: entry c owns the blocks [blk0, blk0 + nblk) of the launch and
+// maps its local block to the single-channel kernel's (x, y).  Gather-part
+// entries start on a multiple of 8 blocks, so a channel's XCD-ordered pair
+// table keeps its XCD assignment.
+template <class T>
+struct KbPartEntry {
   const T* Y;
   int64_t ld;
-  const int* g_ptr;
-  const int* g_node;
+  const int *blk_pair, *pr_lo, *pr_hi, *g_node;
   const T* g_coef;
+  typename acc_of<T>::type* part;
+  int ncol, npair, nbx, blk0, nblk;
+};
+template <class T>
+struct KbReduceEntry {
+  const typename acc_of<T>::type* part;
+  const int* pp_ptr;
   const typename traits<T>::real* gamma;
   T* C;
-  int nproj, blk0;
+  int ncol, nproj, npair, blk0, nblk;
+};
+template <class T>
+struct KbScatterEntry {
+  T* Y;
+  int64_t ld;
+  const int* kb_nodes;
+  const uint32_t* s_pk;
+  const T* s_cph;
+  const int* s_pj;
+  const T* C;
+  typename traits<T>::real alpha;
+  int ncol, nkbn, nproj, nbx, blk0, nblk;
 };
 
-template <class T>
-__global__ void __launch_bounds__(256) kb_gather_batch_kernel(const GatherEntry<T>* __restrict__ E, int nch) {
-  __shared__ T red[256];
+template <class E>
+__device__ __forceinline__ int batch_entry(const E* __restrict__ Es, int nch) {
   int c = 0;
-  while (c + 1 < nch && E[c + 1].blk0 <= (int)blockIdx.x) ++c;
-  const GatherEntry<T>& G = E[c];
+  while (c + 1 < nch && Es[c + 1].blk0 <= (int)blockIdx.x) ++c;
+  return c;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_gather_part_batch_kernel(const KbPartEntry<T>* __restrict__ Es, int nch) {
+  constexpr int K = traits<T>::is_complex ? 2 * GCOL : GCOL;
+  __shared__ double red[(256 / 64) * K];
+  const KbPartEntry<T>& G = Es[batch_entry(Es, nch)];
   const int loc = blockIdx.x - G.blk0;
-  const int p = loc % G.nproj;
-  const int s = loc / G.nproj;
-  const T* y = G.Y + (int64_t)s * G.ld;
-  T acc = zero<T>();
-  const int e1 = G.g_ptr[p + 1];
-  for (int e = G.g_ptr[p] + threadIdx.x; e < e1; e += blockDim.x) acc = cfma(G.g_coef[e], y[G.g_node[e]], acc);
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) G.C[(int64_t)s * G.nproj + p] = rmul(G.gamma[p], red[0]);
+  if (loc >= G.nblk) return;  // padding to the next multiple of 8
+  kb_gather_part_body<T, T>(G.Y, G.ld, G.ncol, G.blk_pair, G.pr_lo, G.pr_hi, G.g_node, G.g_coef, G.npair, G.part,
+                            loc % G.nbx, loc / G.nbx, red);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_gather_reduce_batch_kernel(const KbReduceEntry<T>* __restrict__ Es, int nch) {
+  const KbReduceEntry<T>& G = Es[batch_entry(Es, nch)];
+  kb_gather_reduce_body<T>(G.part, G.ncol, G.nproj, G.npair, G.pp_ptr, G.gamma, G.C, blockIdx.x - G.blk0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) kb_scatter_batch_kernel(const KbScatterEntry<T>* __restrict__ Es, int nch) {
+  extern __shared__ __align__(16) unsigned char ksc_raw[];
+  const KbScatterEntry<T>& S = Es[batch_entry(Es, nch)];
+  const int loc = blockIdx.x - S.blk0;
+  kb_scatter_body<T, T>(S.Y, S.ld, S.ncol, S.kb_nodes, S.nkbn, S.s_pk, S.s_cph, S.s_pj, S.C, S.nproj, S.alpha,
+                        loc % S.nbx, loc / S.nbx, reinterpret_cast<T*>(ksc_raw));
 }
 
 // ---------------------------------------------------------------- launch
@@ -581,15 +644,17 @@ struct cyx_filter_plan_s {
   std::vector<int64_t> ld;
   int tr = 0, flags = 0;
   bool hel = false, kb = false;
-  int nblk = 0, nthr = 0, gblk = 0;
+  int nblk = 0, nthr = 0;
   size_t shm = 0;
-  // per-channel KB coefficients C (ncol x nproj), owned by the plan: the
-  // captured graph holds these addresses for the plan's whole life, so they
-  // must not be the handle's lazily grown scratch
-  std::vector<void*> kbC;
-  void* d_steps = nullptr;   // [m][nch] BatchEntry<T>
-  void* d_gather = nullptr;  // [2][nch] GatherEntry<T>: inputs A, B
-  size_t step_bytes = 0, gather_bytes = 0;
+  // per-channel KB coefficients C (ncol x nproj) and gather partials, owned
+  // by the plan: the captured graph holds these addresses for the plan's
+  // whole life, so they must not be the handle's lazily grown scratch
+  std::vector<void*> kbC, kbPart;
+  void* d_steps = nullptr;   // [m][nch] BatchEntry<T>, then [m][nch] KbScatterEntry<T>
+  void* d_gather = nullptr;  // [2][nch] KbPartEntry<T> (inputs A, B), then [nch] KbReduceEntry<T>
+  size_t step_bytes = 0, gather_bytes = 0, scat_off = 0, red_off = 0;
+  int pblk = 0, rblk = 0, sblk = 0;  // blocks of the KB launches
+  size_t sshm = 0;                   // scatter LDS: KSC x max nproj
   std::vector<unsigned char> h_steps;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -607,8 +672,9 @@ namespace {
 template <class T>
 int plan_fill_steps(cyx_filter_plan_s* pl, const double* a, const double* b, const double* a0) {
   const int nch = pl->nch, m = pl->m;
-  pl->h_steps.assign((size_t)m * nch * sizeof(BatchEntry<T>), 0);
+  pl->h_steps.assign(pl->step_bytes, 0);
   BatchEntry<T>* E = reinterpret_cast<BatchEntry<T>*>(pl->h_steps.data());
+  KbScatterEntry<T>* SE = reinterpret_cast<KbScatterEntry<T>*>(pl->h_steps.data() + pl->scat_off);
   for (int c = 0; c < nch; ++c) {
     CYX_CHECK(b[c] > a[c] && a0[c] < a[c], "channel %d: filter needs a0 < a < b (a0=%g a=%g b=%g)", c, a0[c], a[c],
               b[c]);
@@ -632,8 +698,29 @@ int plan_fill_steps(cyx_filter_plan_s* pl, const double* a, const double* b, con
         sigma = sigma2;
       }
       if (rc != CYX_OK) return rc;
-      L.P.kbC = pl->kb ? (const T*)pl->kbC[c] : nullptr;
+      L.P.kbC = nullptr;  // the KB term is the plan's own gather and scatter launches
       L.P.xcd_remap = 0;  // blocks of a channel do not start on an XCD boundary in general
+      if (pl->kb) {
+        // scatter of this step's output: Y_new += alpha KB C (C gathered from the step's input)
+        const KbTables<T>& K = kb_of<T>(pl->hams[c]);
+        KbScatterEntry<T>& se = SE[(size_t)(i - 1) * nch + c];
+        se.Y = (T*)L.P.out;
+        se.ld = pl->ld[c];
+        se.kb_nodes = K.kb_nodes;
+        se.s_pk = K.s_pk;
+        se.s_cph = K.s_cph;
+        se.s_pj = K.s_pj;
+        se.C = (const T*)pl->kbC[c];
+        se.alpha = L.P.alpha;
+        se.ncol = pl->ncol[c];
+        se.nkbn = K.nkbn;
+        se.nproj = K.nproj;
+        se.nbx = std::max(ceil_div(K.nkbn, 256), 1);
+        se.nblk = se.nbx * ceil_div(pl->ncol[c], KSC);
+        // channel offsets (same for every step)
+        se.blk0 = c == 0 ? 0 : SE[(size_t)(i - 1) * nch + c - 1].blk0 + SE[(size_t)(i - 1) * nch + c - 1].nblk;
+        if (i == 1 && c == nch - 1) pl->sblk = se.blk0 + se.nblk;
+      }
       BatchEntry<T>& be = E[(size_t)(i - 1) * nch + c];
       be.P = L.P;
       be.nblk = (int)L.nblk;
@@ -649,13 +736,16 @@ int plan_fill_steps(cyx_filter_plan_s* pl, const double* a, const double* b, con
 
 template <class T>
 int plan_setup(cyx_filter_plan_s* pl) {
+  using D = typename acc_of<T>::type;
   const int nch = pl->nch;
   int nt = pl->hams[0]->g.nt;
   pl->hel = false;
   pl->kb = false;
   size_t shm = 0;
-  int gblk = 0;
-  std::vector<GatherEntry<T>> G(2 * (size_t)nch);
+  int nproj_max = 0;
+  std::vector<KbPartEntry<T>> GP(2 * (size_t)nch);
+  std::vector<KbReduceEntry<T>> GR((size_t)nch);
+  int pblk = 0, rblk = 0;
   for (int c = 0; c < nch; ++c) {
     cyx_ham_s* h = pl->hams[c];
     CYX_CHECK(h->g.nt == nt, "batched channels must share nt (%d vs %d)", h->g.nt, nt);
@@ -667,49 +757,90 @@ int plan_setup(cyx_filter_plan_s* pl) {
     }
     CYX_CHECK(hel == pl->hel && (K.nproj > 0) == pl->kb,
               "batched channels must all be helical or all cyclic, and all with or without projectors");
-    if (pl->kb && pl->kbC[c] == nullptr)
+    if (pl->kb && pl->kbC[c] == nullptr) {
       CYX_HIP(hipMalloc(&pl->kbC[c], (size_t)pl->ncol[c] * K.nproj * sizeof(T)));
+      CYX_HIP(hipMalloc(&pl->kbPart[c], (size_t)pl->ncol[c] * std::max(K.npair, 1) * sizeof(D)));
+    }
+    nproj_max = std::max(nproj_max, K.nproj);
     const size_t trh = (size_t)stencil_trh<T>(pl->tr);
-    shm = std::max(shm, (2 * stencil_pp<T>() * trh * (nt + 2 * FDN) + (size_t)K.nimgl) * sizeof(T));
+    shm = std::max(shm, 2 * stencil_pp<T>() * trh * (nt + 2 * FDN) * sizeof(T));
+    if (!pl->kb) continue;
+    // two-stage gather: the channel's XCD-ordered pair blocks x column groups,
+    // starting on a multiple of 8 blocks
+    const int nbx = K.nblk_pair, ngrp = ceil_div(pl->ncol[c], GCOL);
     for (int v = 0; v < 2; ++v) {
-      GatherEntry<T>& g = G[(size_t)v * nch + c];
+      KbPartEntry<T>& g = GP[(size_t)v * nch + c];
       g.Y = (const T*)(v == 0 ? pl->A[c] : pl->B[c]);
       g.ld = pl->ld[c];
-      g.g_ptr = K.g_ptr;
+      g.blk_pair = K.blk_pair;
+      g.pr_lo = K.pr_lo;
+      g.pr_hi = K.pr_hi;
       g.g_node = K.g_node;
       g.g_coef = K.g_coef;
-      g.gamma = K.gamma;
-      g.C = (T*)pl->kbC[c];
-      g.nproj = K.nproj;
-      g.blk0 = gblk;
+      g.part = (D*)pl->kbPart[c];
+      g.ncol = pl->ncol[c];
+      g.npair = K.npair;
+      g.nbx = std::max(nbx, 1);
+      g.blk0 = pblk;
+      g.nblk = K.npair > 0 ? nbx * ngrp : 0;
     }
-    gblk += K.nproj * pl->ncol[c];
+    pblk += (GP[c].nblk + 7) & ~7;
+    KbReduceEntry<T>& r = GR[c];
+    r.part = (const D*)pl->kbPart[c];
+    r.pp_ptr = K.pp_ptr;
+    r.gamma = K.gamma;
+    r.C = (T*)pl->kbC[c];
+    r.ncol = pl->ncol[c];
+    r.nproj = K.nproj;
+    r.npair = K.npair;
+    r.blk0 = rblk;
+    r.nblk = ceil_div(pl->ncol[c] * K.nproj, 4);
+    rblk += r.nblk;
   }
-  pl->gblk = gblk;
+  pl->pblk = pblk;
+  pl->rblk = rblk;
+  pl->sshm = (size_t)KSC * nproj_max * sizeof(T);
+  CYX_CHECK(!pl->kb || pl->sshm <= 64 * 1024, "too many projectors for the batched KB scatter (%d)", nproj_max);
   pl->shm = shm;
   pl->nthr = pl->tr * nt;
-  pl->step_bytes = (size_t)pl->m * nch * sizeof(BatchEntry<T>);
-  pl->gather_bytes = 2 * (size_t)nch * sizeof(GatherEntry<T>);
+  pl->scat_off = ((size_t)pl->m * nch * sizeof(BatchEntry<T>) + 255) & ~(size_t)255;
+  pl->step_bytes = pl->scat_off + (pl->kb ? (size_t)pl->m * nch * sizeof(KbScatterEntry<T>) : 0);
+  pl->red_off = (2 * (size_t)nch * sizeof(KbPartEntry<T>) + 255) & ~(size_t)255;
+  pl->gather_bytes = pl->red_off + (size_t)nch * sizeof(KbReduceEntry<T>);
   if (!pl->d_steps) {
     CYX_HIP(hipMalloc(&pl->d_steps, pl->step_bytes));
     CYX_HIP(hipMalloc(&pl->d_gather, pl->gather_bytes));
   }
-  CYX_HIP(hipMemcpy(pl->d_gather, G.data(), pl->gather_bytes, hipMemcpyHostToDevice));
+  std::vector<unsigned char> hg(pl->gather_bytes, 0);
+  std::memcpy(hg.data(), GP.data(), GP.size() * sizeof(KbPartEntry<T>));
+  std::memcpy(hg.data() + pl->red_off, GR.data(), GR.size() * sizeof(KbReduceEntry<T>));
+  CYX_HIP(hipMemcpy(pl->d_gather, hg.data(), pl->gather_bytes, hipMemcpyHostToDevice));
   return CYX_OK;
 }
 
-// the m gathers + stencil steps of one filter pass, enqueued on st
+// the m steps of one filter pass, enqueued on st: per step the two-stage KB
+// gather of the step's input, the stencil step without the KB term, and the
+// KB scatter into its output, each one launch over all channels
 template <class T>
 void plan_enqueue(cyx_filter_plan_s* pl, hipStream_t st, int steps) {
   const BatchEntry<T>* E = (const BatchEntry<T>*)pl->d_steps;
-  const GatherEntry<T>* G = (const GatherEntry<T>*)pl->d_gather;
+  const KbScatterEntry<T>* SE = (const KbScatterEntry<T>*)((const unsigned char*)pl->d_steps + pl->scat_off);
+  const KbPartEntry<T>* GP = (const KbPartEntry<T>*)pl->d_gather;
+  const KbReduceEntry<T>* GR = (const KbReduceEntry<T>*)((const unsigned char*)pl->d_gather + pl->red_off);
+  const int nch = pl->nch;
   // stencil input of step i: A for odd i, B for even i
   for (int i = 1; i <= steps; ++i) {
-    if (pl->kb)
-      hipLaunchKernelGGL(kb_gather_batch_kernel<T>, dim3(pl->gblk), dim3(256), 0, st, G + ((i & 1) ? 0 : pl->nch),
-                         pl->nch);
-    launch_batch<T>(E + (size_t)(i - 1) * pl->nch, pl->nch, pl->nblk, pl->nthr, pl->shm, pl->hel, i > 1, pl->kb,
+    if (pl->kb) {
+      if (pl->pblk > 0)
+        hipLaunchKernelGGL(kb_gather_part_batch_kernel<T>, dim3(pl->pblk), dim3(256), 0, st,
+                           GP + ((i & 1) ? 0 : nch), nch);
+      hipLaunchKernelGGL(kb_gather_reduce_batch_kernel<T>, dim3(pl->rblk), dim3(256), 0, st, GR, nch);
+    }
+    launch_batch<T>(E + (size_t)(i - 1) * nch, nch, pl->nblk, pl->nthr, pl->shm, pl->hel, i > 1, false,
                     stencil_nt_variant(pl->hams[0]->g.nt, pl->tr), st);
+    if (pl->kb && pl->sblk > 0)
+      hipLaunchKernelGGL(kb_scatter_batch_kernel<T>, dim3(pl->sblk), dim3(256), pl->sshm, st,
+                         SE + (size_t)(i - 1) * nch, nch);
   }
 }
 
@@ -856,6 +987,7 @@ int cyx_filter_plan_create(cyx_ctx_t ctx, int nch, const cyx_ham_t* hams, int dt
       }
     pl->hams.push_back(hams[c]);
     pl->kbC.push_back(nullptr);
+    pl->kbPart.push_back(nullptr);
     pl->A.push_back(A[c]);
     pl->B.push_back(B[c]);
     pl->ncol.push_back(ncol[c]);
@@ -872,6 +1004,7 @@ int cyx_filter_plan_create(cyx_ctx_t ctx, int nch, const cyx_ham_t* hams, int dt
     (void)hipFree(pl->d_steps);
     (void)hipFree(pl->d_gather);
     for (void* p : pl->kbC) (void)hipFree(p);
+    for (void* p : pl->kbPart) (void)hipFree(p);
     delete pl;
     return rc;
   }
@@ -929,6 +1062,7 @@ int cyx_filter_plan_destroy(cyx_filter_plan_t pl) {
   (void)hipFree(pl->d_steps);
   (void)hipFree(pl->d_gather);
   for (void* p : pl->kbC) (void)hipFree(p);
+  for (void* p : pl->kbPart) (void)hipFree(p);
   delete pl;
   return CYX_OK;
 }
