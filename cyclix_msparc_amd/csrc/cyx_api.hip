@@ -718,6 +718,11 @@ int cyx_ctx_create(int device, cyx_ctx_t* out) {
   if ((rc0 = noalloc_scratch_init(device)) != CYX_OK) return rc0;
   cyx_ctx_s* c = new cyx_ctx_s();
   c->device = device;
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      c->ncu = ncu;
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     set_error("hipStreamCreate failed");

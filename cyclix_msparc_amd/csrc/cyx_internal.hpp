@@ -101,6 +101,7 @@ enum Phase {
 
 struct cyx_ctx_s {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   bool own_stream = false;
   void* blas = nullptr;  // rocblas_handle

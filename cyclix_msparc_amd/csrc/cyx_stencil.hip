@@ -349,11 +349,27 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   CYX_CHECK(P.tr >= 2 * FDN && P.tr * g.nt <= stencil_max_threads<T>(), "radial tile %d does not fit (nt=%d)", P.tr,
             g.nt);
   P.ntiles_r = ceil_div(g.nr, P.tr);
-  // z planes per workgroup: the whole column unless that leaves fewer than
-  // 1024 workgroups (few columns: 48 complex128 columns at 120x24x96 take
-  // 219 us per launch with 24 planes against 235 us with 48)
+  // z planes per workgroup: the whole column, or z-blocks when few columns
+  // leave compute units idle.  A z-block of zb planes costs zb + 2 FDN
+  // staged planes (its halo), and a launch takes ceil(workgroups / resident
+  // workgroup slots) rounds; the block length that minimises
+  // rounds x (zb + 2 FDN) is taken (ties: the shorter).  Measured at
+  // 120x24x96, complex128 H psi: 17 columns 133 -> 75 us, 8 columns 97 ->
+  // 49 us; 128 and 256 columns keep whole columns.
   int zb = g.nz;
-  while ((long)P.ntiles_r * ceil_div(g.nz, zb) * ncol < 1024 && zb > 4) zb = (zb + 1) / 2;
+  {
+    const long slots = (long)std::max(h->ctx->ncu, 1) * stencil_min_blocks<T>();
+    double best = 1e300;
+    for (int cand = g.nz;; cand = (cand + 1) / 2) {
+      const long nwg = (long)P.ntiles_r * ceil_div(g.nz, cand) * ncol;
+      const double cost = (double)((nwg + slots - 1) / slots) * (cand + 2 * FDN);
+      if (cost <= best) {
+        best = cost;
+        zb = cand;
+      }
+      if (cand <= 4) break;
+    }
+  }
   if (h->tune_zb > 0) zb = std::min(h->tune_zb, g.nz);
   P.zb = zb;
   P.nontemporal = (flags & 1) ? 1 : 0;
