@@ -80,3 +80,32 @@ def test_plan_rejects_mixed_theta_counts(device):
     bl = [torch.zeros((2, s.grid.N), dtype=torch.complex128, device=device.torch_device) for s in (s1, s2)]
     with pytest.raises(CyxError):
         FilterPlan(device, hs, bl, 3)
+
+
+def test_plan_buffers_survive_handle_growth_and_shared_handles(device):
+    """The plan owns its KB coefficients and gather partials: a larger
+    single-channel filter on one of its Hamiltonians afterwards (which grows
+    the handle's own scratch) leaves the plan's next pass exact, and one
+    Hamiltonian may serve two channels of a plan."""
+    from cyclix_msparc_amd.api import FilterPlan
+    hams, blocks = channels(device)
+    hams = [hams[0], hams[1], hams[0]]
+    X2 = torch.from_numpy(np.random.default_rng(21).standard_normal((4, blocks[0].shape[1]))
+                          + 1j * np.random.default_rng(22).standard_normal((4, blocks[0].shape[1])))
+    blocks = [blocks[0], blocks[1], X2.to(device.torch_device)]
+    bounds = ([2.0, 3.0, 2.2], [60.0, 55.0, 65.0], [-1.0, -0.5, -1.5])
+    refs = [h.chebyshev_filter(X, 6, bounds[0][c], bounds[1][c], bounds[2][c]) for c, (h, X) in
+            enumerate(zip(hams, blocks))]
+    work = [X.clone() for X in blocks]
+    plan = FilterPlan(device, hams, work, 6)
+    plan.run(*bounds)
+    for Y, R in zip(work, refs):
+        assert rel(Y, R) < 1e-12
+    big = torch.randn((40, blocks[0].shape[1]), dtype=torch.complex128, device=device.torch_device)
+    hams[0].chebyshev_filter(big, 6, 2.0, 60.0, -1.0)
+    for X, X0 in zip(work, blocks):
+        X.copy_(X0)
+    plan.run(*bounds)
+    for Y, R in zip(work, refs):
+        assert rel(Y, R) < 1e-12
+    plan.close()
