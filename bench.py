@@ -84,6 +84,9 @@ def parse():
                    help="skip the DFPT self-consistency loop on the converged FP64 ground state")
     p.add_argument("--cpu-sample-cols", type=int, default=16)
     p.add_argument("--aar-maxit", type=int, default=1000)
+    p.add_argument("--poisson-precond", default="spectral", choices=("spectral", "jacobi"),
+                   help="AAR preconditioner of the SCF steps' Poisson solves: the exact inverse of Lap_std "
+                        "(in the place of the reference's ILU factors) or its Jacobi diagonal")
     p.add_argument("--poisson-only", action="store_true",
                    help="run only the workload's standalone 1e-8 Poisson solve (cold start) and print its own "
                         "JSON line; for profiling the AAR kernels")
@@ -106,7 +109,9 @@ def run_path(dev, sysm, args, precision, unet=None):
     N, ns, npl = g.N, sysm.nstates, sysm.npl
     bsz = 16 if precision == "fp64" else 8
     # SCF_tol for 1e-5 Ha/atom (src/initialization.m:516-545), poisson_tol = 0.01 SCF_tol
-    opts = SCFOptions(target_energy_accuracy=1e-5, MAXIT_POISSON=args.aar_maxit)
+    # the learned correction rides on Jacobi-AAR
+    pre = "jacobi" if unet is not None else args.poisson_precond
+    opts = SCFOptions(target_energy_accuracy=1e-5, MAXIT_POISSON=args.aar_maxit, poisson_precond=pre)
     gs = GroundState(dev, sysm, options=opts, precision=precision)
     if unet is not None:
         unet.attach(gs.poisson)
@@ -180,6 +185,7 @@ def run_path(dev, sysm, args, precision, unet=None):
         "dtype": "complex128" if precision == "fp64" else
                  "complex64 filter (FP64 Rayleigh-Ritz H-apply, FP64 Gram accumulation, FP64 dense eig/energies)",
         "learned_poisson": unet is not None,
+        "poisson_precond": "jacobi" if unet is not None else args.poisson_precond,
         "roofline": {"bound": "hbm", "kernel": f"stencil_step_kernel<{'c128' if precision == 'fp64' else 'c64'}> "
                                                "(fused local H-apply + Chebyshev step; the KB term is its own gather/scatter launches)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -418,7 +424,8 @@ def main():
     rho_at = torch.from_numpy(pc["rho_at"]).to(tdev)
     poisson = Poisson(dev, g)
     pois = {}
-    for mode in (["jacobi", "learned"] if unet is not None else ["jacobi"]):
+    for mode in ["spectral", "jacobi"] + (["learned"] if unet is not None else []):
+        poisson.set_spectral(mode == "spectral")
         if mode == "learned":
             unet.attach(poisson)
         poisson.solve(rho_at, b, None, 1e-8, 20)
@@ -486,7 +493,8 @@ def main():
         "config": {"workload": f"{args.workload}: (10,0) CNT 1/5 sector, twist 0.05, 8 atoms, grid "
                                f"{g.nr}x{g.nt}x{g.nz} (N={N}), {ns} states, Chebyshev degree {npl}, "
                                "1 Bloch channel; SCF iterations with Periodic-Pulay potential mixing at the "
-                               "SCF_tol of 1e-5 Ha/atom, AAR Poisson to 0.01 SCF_tol",
+                               "SCF_tol of 1e-5 Ha/atom, AAR Poisson to 0.01 SCF_tol "
+                               f"({args.poisson_precond} preconditioner)",
                    "path": {"fp64": "FP64 GPU", "mixed": "mixed precision" + (" + learned Poisson correction"
                                                                               if args.learned and unet is not None
                                                                               else "")}[head]},
@@ -517,8 +525,10 @@ def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args):
     """The plain-C FP64 oracle (OpenMP) on a bounded sample of the same SCF
     step, scaled to one step: the degree-npl filter on `cpu-sample-cols`
     columns (scaled by ns / cols), the three Rayleigh-Ritz GEMMs with numpy
-    at N/16 rows (scaled by 16) plus one H psi, and 20 Jacobi-AAR iterations
-    (scaled to the GPU's iteration count)."""
+    at N/16 rows (scaled by 16) plus one H psi, and the Poisson solve: 20
+    Jacobi-AAR iterations scaled to the GPU's iteration count, or with the
+    spectral preconditioner the oracle's whole solve (numpy FFT and per-mode
+    inverses, which the step's count of about 7 iterations applies)."""
     from oracle import c_oracle
     from oracle import cyclix_oracle as O
     g = sysm.grid
@@ -540,14 +550,22 @@ def cpu_baseline(sysm, proj, pc, veff, state, ms_step, args):
     _ = A @ M
     t_rr = (time.perf_counter() - t) * 16 + t_filter / npl  # + one H psi
     f = O.poisson_rhs(pc["rho_at"], pc["b"])
-    t = time.perf_counter()
-    Cc.aar_jacobi(f, None, 1e-30, 20)
-    t_aar = (time.perf_counter() - t) / 20 * max(state["aar_its"], 1.0)
+    if args.poisson_precond == "spectral":
+        O.spectral_inverses(og)  # set-up, once per grid (not per step)
+        t = time.perf_counter()
+        _, _, its = O.poisson_solve(og, pc["rho_at"], pc["b"], 1e-8, None, 1000, "spectral")
+        t_aar = (time.perf_counter() - t) / max(its, 1) * max(state["aar_its"], 1.0)
+        aar_note = f"the oracle's spectral AAR solve ({its} iterations, numpy)"
+    else:
+        t = time.perf_counter()
+        Cc.aar_jacobi(f, None, 1e-30, 20)
+        t_aar = (time.perf_counter() - t) / 20 * max(state["aar_its"], 1.0)
+        aar_note = "20 Jacobi-AAR iterations"
     total = t_filter + t_rr + t_aar
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": round(total * 1e3, 1), "unit": "ms/SCF-step", "cores": cores, "kind": "port",
             "sample": f"C FP64 oracle (OpenMP, {cores} threads): filter on {c} of {ns} columns (degree {npl}), "
-                      f"numpy RR GEMMs at N/16, 20 Jacobi-AAR iterations; each scaled to the full step "
+                      f"numpy RR GEMMs at N/16, {aar_note}; each scaled to the full step "
                       f"(filter {t_filter:.1f} s, RR {t_rr:.1f} s, AAR {t_aar:.1f} s)",
             "gpu_speedup": round(total * 1e3 / ms_step, 1)}
 

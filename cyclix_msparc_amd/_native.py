@@ -31,7 +31,7 @@ EXPORTS = (
     "cyx_sternheimer_plan_status", "cyx_sternheimer_plan_destroy", "cyx_sternheimer_rhs",
     "cyx_unet_create", "cyx_unet_destroy", "cyx_unet_num_params", "cyx_unet_set_params", "cyx_unet_get_params",
     "cyx_unet_forward", "cyx_unet_gradient", "cyx_unet_train_step", "cyx_unet_input",
-    "cyx_poisson_set_preconditioner", "cyx_poisson_apply",
+    "cyx_poisson_set_preconditioner", "cyx_poisson_apply", "cyx_poisson_set_spectral", "cyx_poisson_spectral_apply",
     "cyx_filter_plan_create", "cyx_filter_plan_run", "cyx_filter_plan_tuning", "cyx_filter_plan_destroy",
     "cyx_filter_plan_attach_sternheimer",
 )
@@ -132,6 +132,8 @@ def load():
         "cyx_unet_input": (i32, [vp, vp, vp, vp, i32, vp]),
         "cyx_poisson_set_preconditioner": (i32, [vp, vp]),
         "cyx_poisson_apply": (i32, [vp, vp, vp]),
+        "cyx_poisson_set_spectral": (i32, [vp, i32]),
+        "cyx_poisson_spectral_apply": (i32, [vp, vp, vp, i32, i64]),
         "cyx_filter_plan_create": (i32, [vp, i32, C.POINTER(vp), i32, C.POINTER(vp), C.POINTER(vp),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int64), i32, i32, pd, pd, pd,
                                          C.POINTER(vp)]),

@@ -356,7 +356,7 @@ class Hamiltonian:
                                                  C.c_void_p(X.data_ptr()), B.shape[0], self.N, nat.dptr(lam),
                                                  C.c_void_p(psi_v.data_ptr()) if nv else None, self.N, nv,
                                                  1 if X0 is not None else 0, float(tol), int(max_iter),
-                                                 1 if precond else 0, C.byref(info), nat.iptr(its),
+                                                 precond_code(precond), C.byref(info), nat.iptr(its),
                                                  nat.dptr(rel)), allow=allow)
         return X, info, its, rel
 
@@ -435,7 +435,7 @@ class SternheimerPlan:
                                                        C.c_void_p(self.X.data_ptr()), self.ncol, ham.N,
                                                        nat.dptr(lam), C.c_void_p(psi_v.data_ptr()) if nv else None,
                                                        ham.N, nv, 1 if X0 is not None else 0, self.tol,
-                                                       1 if precond else 0, C.byref(h)))
+                                                       precond_code(precond), C.byref(h)))
         self.h = h
         self.dev.register(self)
 
@@ -625,9 +625,12 @@ class Mixer:
 
 
 class Poisson:
-    """``S.Lap_std`` (k = 0) with a Jacobi preconditioner, on the device."""
+    """``S.Lap_std`` (k = 0) on the device, with the Jacobi diagonal or, with
+    ``spectral=True``, the exact inverse of Lap_std (theta/z FFT and one band
+    solve per mode) as the AAR preconditioner, in the place of the
+    reference's ILU factors (``src/poissonSolve.m:36``)."""
 
-    def __init__(self, dev: Device, grid: GridSpec):
+    def __init__(self, dev: Device, grid: GridSpec, spectral=False):
         self.dev = dev
         self.lib = dev.lib
         self.N = grid.N
@@ -636,6 +639,22 @@ class Poisson:
         nat.check(self.lib.cyx_poisson_create(dev.h, C.byref(self._gd), C.byref(h)))
         self.h = h
         self.dev.register(self)
+        self.spectral = False
+        if spectral:
+            self.set_spectral(True)
+
+    def set_spectral(self, on=True):
+        nat.check(self.lib.cyx_poisson_set_spectral(self.h, 1 if on else 0))
+        self.spectral = bool(on)
+
+    def spectral_apply(self, r):
+        """z = Lap_std^-1 r exactly (FP64, (N,) or (ncol, N))."""
+        r = r.contiguous()
+        z = torch.empty_like(r)
+        ncol = 1 if r.dim() == 1 else r.shape[0]
+        nat.check(self.lib.cyx_poisson_spectral_apply(self.h, C.c_void_p(r.data_ptr()), C.c_void_p(z.data_ptr()),
+                                                      ncol, self.N))
+        return z
 
     def close(self):
         if getattr(self, "h", None):
@@ -675,6 +694,19 @@ class Poisson:
                                              1 if phi_guess is not None else 0, float(tol), int(max_iter),
                                              C.byref(info)), allow=(nat.CYX_ERR_NOCONV,))
         return phi, info
+
+
+def precond_code(precond):
+    """The Sternheimer preconditioner argument of the C-ABI: "spectral" (2,
+    the exact Lap_std^-1 in the place of the reference's ILU factors),
+    "jacobi" or True (1), "none" or False (0)."""
+    if precond is True or precond == "jacobi":
+        return 1
+    if precond is False or precond is None or precond == "none":
+        return 0
+    if precond == "spectral":
+        return 2
+    raise ValueError(f"precond {precond!r}: 'spectral', 'jacobi' or 'none'")
 
 
 # --------------------------------------------------------------------------

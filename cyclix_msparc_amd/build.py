@@ -69,7 +69,7 @@ def build(verbose: bool = False, extra=()) -> Path:
                 print(msg, file=sys.stderr)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp),
-           f"-L{ROCM / 'lib'}", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM / 'lib'}"]
+           f"-L{ROCM / 'lib'}", "-lrocsolver", "-lrocblas", "-lrocfft", f"-Wl,-rpath,{ROCM / 'lib'}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -787,10 +787,277 @@ static int run_impl(cyx_poisson_s* ps, const double* f, double* x, int use_guess
   return hs.done == 1 ? CYX_OK : CYX_ERR_NOCONV;
 }
 
+// ---- AAR with the exact inverse of Lap_std as its preconditioner (the place
+// of the reference's ILU factors, src/poissonSolve.m:36).  The iteration is
+// src/aar.m:33-72 step by step: the residual of x_prev and its norm, the
+// preconditioned residual, the history columns, a Richardson or an Anderson
+// update; the iterate returned is the one whose residual met the tolerance.
+// With M = Lap_std^-1 the preconditioned residual is the error itself, so the
+// first Anderson step (count = p) lands on the solution: a solve takes p + 1
+// iterations at any tolerance.  The host reads one norm per iteration and,
+// at Anderson steps, the m(m+1)/2 + m history products (gamma = pinv(DF'DF)
+// DF'res with MATLAB's tolerance, on the host: m <= 8).
+
+// r = f - Lap x; per-block partial sums of r^2
+__global__ void __launch_bounds__(PB) sp_resid_kernel(PoissonOp P, const double* __restrict__ f,
+                                                      const double* __restrict__ x, const double* __restrict__ T,
+                                                      double* __restrict__ r, double* __restrict__ part) {
+  __shared__ double red[PB / 64];
+  const Pt q = point_of(P);
+  double v = 0.0;
+  if (q.ok) {
+    v = f[q.n] - lap_at(P, x, T, q);
+    r[q.n] = v;
+  }
+  double vv[1] = {v * v};
+  const double s = block_sums<1, PB>(vv, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// history columns DX_k = x_prev - Xold, DF_k = res - Fold (hist), then
+// Xold = x_prev, Fold = res; Richardson x_new = x_prev + omega res (rich)
+__global__ void __launch_bounds__(256) sp_hist_kernel(int64_t N, const double* __restrict__ res,
+                                                      const double* __restrict__ xp, double* __restrict__ Xold,
+                                                      double* __restrict__ Fold, double* __restrict__ DXk,
+                                                      double* __restrict__ DFk, int hist, int rich, double omega,
+                                                      double* __restrict__ xn) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  const double x = xp[n], rs = res[n];
+  if (hist) {
+    DXk[n] = x - Xold[n];
+    DFk[n] = rs - Fold[n];
+  }
+  Xold[n] = x;
+  Fold[n] = rs;
+  if (rich) xn[n] = __builtin_fma(omega, rs, x);
+}
+
+// per-block partials of DF_a . DF_b (a <= b, row-major upper triangle) and
+// DF_a . res: part[blk * nd + d]
+template <int M>
+__global__ void __launch_bounds__(256) sp_dots_kernel(int64_t N, const double* __restrict__ DF,
+                                                      const double* __restrict__ res, double* __restrict__ part) {
+  constexpr int ND = M * (M + 1) / 2 + M;
+  __shared__ double red[(256 / 64) * ND];
+  double v[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    double c[M];
+#pragma unroll
+    for (int a = 0; a < M; ++a) c[a] = DF[(int64_t)a * N + n];
+    const double rs = res[n];
+    int d = 0;
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+      for (int b = a; b < M; ++b) {
+        v[d] = __builtin_fma(c[a], c[b], v[d]);
+        ++d;
+      }
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+      v[d] = __builtin_fma(c[a], rs, v[d]);
+      ++d;
+    }
+  }
+  const double s = block_sums<ND, 256>(v, red);
+  if (threadIdx.x < ND) part[(int64_t)blockIdx.x * ND + threadIdx.x] = s;
+}
+
+// x_new = x_prev + beta res - sum_k gamma_k (DX_k + beta DF_k)
+__global__ void __launch_bounds__(256) sp_anderson_kernel(int64_t N, int m, const double* __restrict__ xp,
+                                                          const double* __restrict__ res,
+                                                          const double* __restrict__ DX,
+                                                          const double* __restrict__ DF, const double* __restrict__ g,
+                                                          double beta, double* __restrict__ xn) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  double acc = __builtin_fma(beta, res[n], xp[n]);
+  for (int k = 0; k < m; ++k) acc -= g[k] * __builtin_fma(beta, DF[(int64_t)k * N + n], DX[(int64_t)k * N + n]);
+  xn[n] = acc;
+}
+
+// gamma = pinv(A) b for the symmetric m x m A (row-major), MATLAB pinv's
+// tolerance max(m, m) * norm(A) * eps, by cyclic Jacobi; returns whether an
+// eigenvalue was cut
+static bool host_pinv_solve(int m, std::vector<double> A, const double* b, double* g) {
+  std::vector<double> V((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) (i == j ? tot : off) += A[(size_t)i * m + j] * A[(size_t)i * m + j];
+    if (off <= 1e-32 * tot || off == 0.0) break;
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = A[(size_t)p * m + q];
+        if (apq == 0.0) continue;
+        const double th = 0.5 * (A[(size_t)q * m + q] - A[(size_t)p * m + p]) / apq;
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < m; ++k) {  // A <- J' A J
+          const double akp = A[(size_t)k * m + p], akq = A[(size_t)k * m + q];
+          A[(size_t)k * m + p] = c * akp - sn * akq;
+          A[(size_t)k * m + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double apk = A[(size_t)p * m + k], aqk = A[(size_t)q * m + k];
+          A[(size_t)p * m + k] = c * apk - sn * aqk;
+          A[(size_t)q * m + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < m; ++k) {
+          const double vkp = V[(size_t)k * m + p], vkq = V[(size_t)k * m + q];
+          V[(size_t)k * m + p] = c * vkp - sn * vkq;
+          V[(size_t)k * m + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  double nrm = 0.0;
+  for (int i = 0; i < m; ++i) nrm = std::max(nrm, std::fabs(A[(size_t)i * m + i]));
+  const double tol = m * nrm * 2.220446049250313e-16;
+  bool cut = false;
+  for (int i = 0; i < m; ++i) g[i] = 0.0;
+  for (int e = 0; e < m; ++e) {
+    const double lam = A[(size_t)e * m + e];
+    if (std::fabs(lam) <= tol) {
+      cut = true;
+      continue;
+    }
+    double vb = 0.0;
+    for (int i = 0; i < m; ++i) vb += V[(size_t)i * m + e] * b[i];
+    for (int i = 0; i < m; ++i) g[i] += V[(size_t)i * m + e] * vb / lam;
+  }
+  return cut;
+}
+
+static int run_spectral(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
+                        double omega, double beta, int m, int p, cyx_aar_info* info) {
+  CYX_CHECK(m >= 1 && m <= 8, "AAR history m must be in [1, 8]");
+  CYX_CHECK(p >= 1 && max_iter >= 1 && tol > 0, "invalid AAR parameters");
+  const int64_t N = ps->N();
+  CYX_CHECK(N < ((int64_t)1 << 31) - PB, "grid too large for 32-bit point indices");
+  hipStream_t st = ps->ctx->stream;
+  const PoissonOp P = make_op(ps);
+  const int nblk = (int)ceil_div(N, PB);
+  const int ndot = 1024;  // blocks of the history products
+  const int nd = m * (m + 1) / 2 + m;
+  // scratch: xa | xb | r | res | Xold | Fold | T | DX | DF | partials | gamma
+  const size_t vec = (size_t)N * sizeof(double);
+  const size_t npart = (size_t)std::max(nblk, ndot * nd) + 64;
+  const size_t need = vec * (7 + 2 * (size_t)m) + npart * sizeof(double) + 8 * sizeof(double) + 256;
+  if (ps->scratch_bytes < need) {
+    if (ps->scratch) (void)hipFree(ps->scratch);
+    ps->scratch = nullptr;
+    ps->scratch_bytes = 0;
+    if (hipMalloc((void**)&ps->scratch, need) != hipSuccess) {
+      set_error("poisson scratch allocation of %zu bytes failed", need);
+      return CYX_ERR_NOMEM;
+    }
+    ps->scratch_bytes = need;
+  }
+  double* xa = ps->scratch;
+  double* r = xa + N;
+  double* res = r + N;
+  double* Xold = res + N;
+  double* Fold = Xold + N;
+  double* T = Fold + N;
+  double* DX = T + N;
+  double* DF = DX + (int64_t)m * N;
+  double* part = DF + (int64_t)m * N;
+  double* gdev = part + npart;
+  // x_prev alternates between the caller's x and xa
+  double* X[2] = {x, xa};
+  const dim3 gv(ceil_div(N, 256));
+  if (!use_guess) hipLaunchKernelGGL(fill_kernel, gv, dim3(256), 0, st, x, N, 1.0);
+  CYX_HIP(hipMemsetAsync(DX, 0, vec * 2 * m, st));
+  CYX_HIP(hipMemsetAsync(T, 0, vec, st));
+  std::vector<double> hp(std::max((size_t)nblk, (size_t)ndot * nd));
+  // ||f||
+  hipLaunchKernelGGL(norm_kernel, dim3(nblk < 1024 ? nblk : 1024), dim3(AB), 0, st, f, N, part);
+  const int nnorm = nblk < 1024 ? nblk : 1024;
+  CYX_HIP(hipMemcpyAsync(hp.data(), part, (size_t)nnorm * sizeof(double), hipMemcpyDeviceToHost, st));
+  CYX_HIP(hipStreamSynchronize(st));
+  double fn2 = 0.0;
+  for (int b = 0; b < nnorm; ++b) fn2 += hp[b];
+  CYX_CHECK(fn2 > 0.0, "AAR right-hand side is zero");
+  const double nb = 1.0 / std::sqrt(fn2);
+  PhaseTimer tm(ps->ctx, PH_AAR);
+  int count = 1, cur = 0, pinv_cut = 0;
+  double relres = 1.0 + tol;
+  std::vector<double> A((size_t)m * m), bv(m), gam(m);
+  while (count <= max_iter) {
+    double* xp = X[cur];
+    double* xn = X[cur ^ 1];
+    if (P.helical) {
+      const ThetaShape ts = theta_shape(P, xp, T);
+      hipLaunchKernelGGL(theta_kernel, dim3(ts.ntb), dim3(PB), 0, st, P, (const double*)xp, T, ts.ntb, ts.pair,
+                         FinArgs{});
+    }
+    hipLaunchKernelGGL(sp_resid_kernel, dim3(nblk), dim3(PB), 0, st, P, f, (const double*)xp, (const double*)T, r,
+                       part);
+    CYX_HIP(hipGetLastError());
+    CYX_HIP(hipMemcpyAsync(hp.data(), part, (size_t)nblk * sizeof(double), hipMemcpyDeviceToHost, st));
+    CYX_HIP(hipStreamSynchronize(st));
+    double r2 = 0.0;
+    for (int b = 0; b < nblk; ++b) r2 += hp[b];
+    relres = std::sqrt(r2) * nb;
+    if (relres <= tol) break;  // x_prev is the iterate src/aar.m returns
+    if (count == max_iter) break;
+    int rc = spectral_solve_real(ps->spec, r, res, 1, N, st);
+    if (rc != CYX_OK) return rc;
+    const int anderson = (count % p == 0) ? 1 : 0;
+    const int k = count > 1 ? (count - 2) % m : 0;
+    hipLaunchKernelGGL(sp_hist_kernel, gv, dim3(256), 0, st, N, (const double*)res, (const double*)xp, Xold, Fold,
+                       DX + (int64_t)k * N, DF + (int64_t)k * N, count > 1 ? 1 : 0, anderson ? 0 : 1, omega, xn);
+    if (anderson) {
+      switch (m) {
+#define CYX_SPD(MM) \
+  case MM: hipLaunchKernelGGL(sp_dots_kernel<MM>, dim3(ndot), dim3(256), 0, st, N, (const double*)DF, (const double*)res, part); break;
+        CYX_SPD(1) CYX_SPD(2) CYX_SPD(3) CYX_SPD(4) CYX_SPD(5) CYX_SPD(6) CYX_SPD(7) CYX_SPD(8)
+#undef CYX_SPD
+      }
+      CYX_HIP(hipMemcpyAsync(hp.data(), part, (size_t)ndot * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+      CYX_HIP(hipStreamSynchronize(st));
+      std::vector<double> tot(nd, 0.0);
+      for (int b = 0; b < ndot; ++b)
+        for (int d = 0; d < nd; ++d) tot[d] += hp[(size_t)b * nd + d];
+      int d = 0;
+      for (int a = 0; a < m; ++a)
+        for (int b2 = a; b2 < m; ++b2) {
+          A[(size_t)a * m + b2] = tot[d];
+          A[(size_t)b2 * m + a] = tot[d];
+          ++d;
+        }
+      for (int a = 0; a < m; ++a) bv[a] = tot[d++];
+      pinv_cut += host_pinv_solve(m, A, bv.data(), gam.data()) ? 1 : 0;
+      CYX_HIP(hipMemcpyAsync(gdev, gam.data(), (size_t)m * sizeof(double), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(sp_anderson_kernel, gv, dim3(256), 0, st, N, m, (const double*)xp, (const double*)res,
+                         (const double*)DX, (const double*)DF, (const double*)gdev, beta, xn);
+    }
+    CYX_HIP(hipGetLastError());
+    cur ^= 1;
+    ++count;
+  }
+  if (X[cur] != x) CYX_HIP(hipMemcpyAsync(x, X[cur], vec, hipMemcpyDeviceToDevice, st));
+  CYX_HIP(hipStreamSynchronize(st));
+  const bool conv = relres <= tol;
+  if (info) {
+    info->iterations = count;
+    info->relres = relres;
+    info->converged = conv ? 1 : 0;
+    info->precond = 3;
+    info->pinv_eig = pinv_cut;
+  }
+  return conv ? CYX_OK : CYX_ERR_NOCONV;
+}
+
 int poisson_run(cyx_poisson_s* ps, const double* f, double* x, int use_guess, double tol, int max_iter,
                 double omega, double beta, int m, int p, cyx_aar_info* info) {
   bool diverged = false;
   cyx_unet_s* net = ps->net;
+  if (ps->spec && ps->use_spectral) return run_spectral(ps, f, x, use_guess, tol, max_iter, omega, beta, m, p, info);
   if (!net) return run_impl(ps, f, x, use_guess, tol, max_iter, omega, beta, m, p, info, nullptr, &diverged);
   // keep the guess: a diverging learned solve is restarted with Jacobi
   const int64_t N = ps->N();

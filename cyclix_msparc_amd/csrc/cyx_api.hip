@@ -70,12 +70,15 @@ static void fd_weights(int fdn, double* w2, double* w1) {
   }
 }
 
-static int check_grid(const cyx_grid_desc* g) {
+// stencil = true: the grid a Hamiltonian handle runs the marching stencil on
+// (theta rings up to MAX_NT nodes); the Poisson operator reads its theta taps
+// from memory and takes any ring
+static int check_grid(const cyx_grid_desc* g, bool stencil = true) {
   CYX_CHECK(g != nullptr, "grid is null");
   CYX_CHECK(g->fdn == FDN, "only FDn = %d is supported (got %d)", FDN, g->fdn);
   CYX_CHECK(g->nr >= 1 && g->nt >= FDN && g->nz >= FDN, "grid %dx%dx%d too small for FDn=%d", g->nr, g->nt, g->nz,
             FDN);
-  CYX_CHECK(g->nt <= 512 / (2 * FDN), "nt = %d exceeds %d", g->nt, 512 / (2 * FDN));
+  CYX_CHECK(!stencil || g->nt <= MAX_NT, "nt = %d exceeds %d", g->nt, MAX_NT);
   CYX_CHECK(g->dr > 0 && g->dt > 0 && g->dz > 0, "spacings must be positive");
   CYX_CHECK(g->cell_typ == 3 || g->cell_typ == 4 || g->cell_typ == 5, "cell_typ must be 3, 4 or 5");
   CYX_CHECK(g->cell_typ == 3 || g->twist != 0.0, "helical cell_typ %d needs a non-zero twist", g->cell_typ);
@@ -922,6 +925,7 @@ int cyx_ham_destroy(cyx_ham_t h) {
   (void)hipFree(h->sub.p);
   (void)hipFree(h->red.p);
   (void)hipFree(h->stern.p);
+  cyx::spectral_destroy(h->spec);
   delete h;
   return CYX_OK;
 }
@@ -1125,7 +1129,7 @@ int cyx_energy_terms(cyx_ham_t h, const double* rho, const double* vxc, const do
 int cyx_poisson_create(cyx_ctx_t ctx, const cyx_grid_desc* grid, cyx_poisson_t* out) {
   CYX_CHECK(ctx && out, "null argument");
   int rc;
-  if ((rc = check_grid(grid)) != CYX_OK) return rc;
+  if ((rc = check_grid(grid, false)) != CYX_OK) return rc;
   CYX_HIP(hipSetDevice(ctx->device));
   cyx_poisson_s* p = new cyx_poisson_s();
   p->ctx = ctx;
@@ -1173,12 +1177,35 @@ int cyx_poisson_set_preconditioner(cyx_poisson_t p, cyx_unet_t net) {
 int cyx_poisson_destroy(cyx_poisson_t p) {
   if (!p) return CYX_OK;
   (void)hipDeviceSynchronize();
+  cyx::spectral_destroy(p->spec);
   (void)hipFree(p->coef);
   (void)hipFree(p->coefT);
   (void)hipFree(p->scratch);
   (void)hipFree(p->tbuf.p);
   delete p;
   return CYX_OK;
+}
+
+int cyx_poisson_set_spectral(cyx_poisson_t ps, int on) {
+  CYX_CHECK(ps, "null argument");
+  CYX_HIP(hipSetDevice(ps->ctx->device));
+  if (on && !ps->spec) {
+    int rc = cyx::spectral_create(ps->g, &ps->spec);
+    if (rc != CYX_OK) return rc;
+  }
+  ps->use_spectral = on != 0;
+  return CYX_OK;
+}
+
+int cyx_poisson_spectral_apply(cyx_poisson_t ps, const double* r, double* z, int ncol, int64_t ld) {
+  CYX_CHECK(ps && r && z, "null argument");
+  CYX_CHECK(ncol >= 1 && ld >= ps->N(), "invalid block shape (ncol %d, ld %lld)", ncol, (long long)ld);
+  CYX_HIP(hipSetDevice(ps->ctx->device));
+  if (!ps->spec) {
+    int rc = cyx::spectral_create(ps->g, &ps->spec);
+    if (rc != CYX_OK) return rc;
+  }
+  return cyx::spectral_solve_real(ps->spec, r, z, ncol, ld, ps->ctx->stream);
 }
 
 int cyx_aar(cyx_poisson_t ps, const double* f, double* x, int use_guess, double tol, int max_iter, double omega,

@@ -12,6 +12,7 @@ namespace cyx {
 constexpr int FDN = 6;            // finite-difference half-width the kernels are built for
 constexpr int QN = 2 * FDN + 1;   // z queue depth of the marching stencil
 constexpr int RCOEF = 2 * FDN + 2;  // per-radius coefficients: c+[6], c-[6], c_theta, c_diag
+constexpr int MAX_NT = 128;        // theta ring limit of the marching stencil (512 threads, radial tiles >= 4)
 
 // order-2 FDN central weights, src/initialization.m:403-419: w2 second and
 // w1 first derivative, index 0 the centre
@@ -136,6 +137,17 @@ struct PhaseTimer {
 };
 }  // namespace cyx
 
+namespace cyx {
+// exact inverse of Lap_std (k = 0) by a theta/z FFT and one band solve per mode
+struct Spectral;
+int spectral_create(const cyx_grid_desc& g, Spectral** out);
+void spectral_destroy(Spectral* S);
+size_t spectral_factor_bytes(const Spectral* S);
+// z = Lap_std^-1 r for ncol columns (leading dimension ld) on stream st
+int spectral_solve_real(Spectral* S, const double* r, double* z, int ncol, int64_t ld, hipStream_t st);
+int spectral_solve_complex(Spectral* S, const cplx<double>* r, cplx<double>* z, int ncol, int64_t ld, hipStream_t st);
+}  // namespace cyx
+
 struct cyx_ham_s {
   cyx_ctx_s* ctx = nullptr;
   int device = 0;
@@ -163,12 +175,15 @@ struct cyx_ham_s {
   cyx::DeviceBuf sub;      // M, Hs, eigenvalues, solver work/info, column scales
   cyx::DeviceBuf red;      // per-block partial sums of the energy and SCF-error reductions
   cyx::DeviceBuf stern;    // Sternheimer operator / BiCGSTAB workspace
+  cyx::Spectral* spec = nullptr;  // exact Lap_std^-1 of the grid (Sternheimer preconditioner), built on first use
   int64_t N() const { return (int64_t)g.nr * g.nt * g.nz; }
 };
 
 struct cyx_poisson_s {
   cyx_ctx_s* ctx = nullptr;
   cyx_unet_s* net = nullptr;  // learned preconditioner (null: Jacobi only)
+  cyx::Spectral* spec = nullptr;  // exact Lap_std^-1 (built by cyx_poisson_set_spectral)
+  bool use_spectral = false;      // AAR preconditioned by spec instead of the Jacobi diagonal
   cyx_grid_desc g{};
   double* coef = nullptr;   // [nr][RCOEF] Lap_std coefficients (no -1/2)
   double* coefT = nullptr;  // [RCOEF][nr] the same, radius fastest, taps outside [0, nr) zeroed
@@ -281,5 +296,7 @@ size_t dtype_size(int dtype);
 // meanwhile, so that the launches can be captured into a caller's graph)
 int stern_plan_enqueue(cyx_stern_plan_s* sp, hipStream_t st, int iters);
 cyx_ctx_s* stern_plan_context(const cyx_stern_plan_s* sp);
+// whether the plan preconditions with the exact Lap_std^-1 (its iterations are not captured)
+bool stern_plan_spectral(const cyx_stern_plan_s* sp);
 
 }  // namespace cyx

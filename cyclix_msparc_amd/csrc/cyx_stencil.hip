@@ -306,14 +306,17 @@ const OpCoef<float>& coef_of<float>(cyx_ham_s* h) {
   return h->c32;
 }
 
-// r tile: largest tile with tr * nt <= maxthr and tr >= 2*FDN, balanced over nr
+// r tile: largest tile with tr * nt <= maxthr, balanced over nr; at least
+// 2*FDN (one right-halo load per thread) unless the theta ring is too wide
+// for that, then at least STENCIL_TR_MIN (the narrow-tile variant)
 static int choose_tr(int nr, int nt, int maxthr) {
   int maxtr = maxthr / nt;
   if (maxtr > nr) maxtr = nr;
-  if (maxtr < 2 * FDN) maxtr = 2 * FDN;
+  const int lo = maxthr / nt >= 2 * FDN ? 2 * FDN : STENCIL_TR_MIN;
+  if (maxtr < lo) maxtr = lo;
   int ntiles = (nr + maxtr - 1) / maxtr;
   int tr = (nr + ntiles - 1) / ntiles;
-  if (tr < 2 * FDN) tr = 2 * FDN;
+  if (tr < lo) tr = lo;
   return tr;
 }
 
@@ -344,10 +347,11 @@ static int make_step(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   P.nt = g.nt;
   P.nz = g.nz;
   P.tr = choose_tr(g.nr, g.nt, stencil_max_threads<T>());
-  if (h->tune_tr > 0) P.tr = std::max(2 * FDN, std::min(h->tune_tr, stencil_max_threads<T>() / g.nt));
+  if (h->tune_tr > 0 && stencil_max_threads<T>() / g.nt >= 2 * FDN)
+    P.tr = std::max(2 * FDN, std::min(h->tune_tr, stencil_max_threads<T>() / g.nt));
   if (tr_force > 0) P.tr = tr_force;
-  CYX_CHECK(P.tr >= 2 * FDN && P.tr * g.nt <= stencil_max_threads<T>(), "radial tile %d does not fit (nt=%d)", P.tr,
-            g.nt);
+  CYX_CHECK(P.tr >= STENCIL_TR_MIN && P.tr * g.nt <= stencil_max_threads<T>(), "radial tile %d does not fit (nt=%d)",
+            P.tr, g.nt);
   P.ntiles_r = ceil_div(g.nr, P.tr);
   // z planes per workgroup: the whole column, or z-blocks when few columns
   // leave compute units idle.  A z-block of zb planes costs zb + 2 FDN
@@ -511,7 +515,11 @@ static int step_impl(cyx_ham_s* h, const void* in_, const void* prev_, void* out
   const KbTables<T>& K = kb_of<T>(h);
   // (systems with so many projectors that the scatter's LDS slice of C would
   // not fit keep the scatter inside the stencil)
-  const bool kb = L.kb && (size_t)KSC * K.nproj * sizeof(T) <= 32 * 1024;
+  const bool fits = (size_t)KSC * K.nproj * sizeof(T) <= 32 * 1024;
+  // (the narrow-tile variant of wide theta rings has no KB term inside)
+  CYX_CHECK(!L.kb || fits || L.P.tr >= 2 * FDN, "%d projectors on a theta ring of %d nodes are not supported", K.nproj,
+            h->g.nt);
+  const bool kb = L.kb && fits;
   if (kb) L.kb = false;
   {
     PhaseTimer tm(h->ctx, phase >= 0 ? phase : (prev_ ? PH_STENCIL : PH_HAPPLY));
@@ -762,6 +770,8 @@ int plan_setup(cyx_filter_plan_s* pl) {
   std::vector<KbPartEntry<T>> GP(2 * (size_t)nch);
   std::vector<KbReduceEntry<T>> GR((size_t)nch);
   int pblk = 0, rblk = 0;
+  CYX_CHECK(pl->tr >= 2 * FDN, "batched plans take theta rings of at most %d nodes (got %d)",
+            stencil_max_threads<T>() / (2 * FDN), nt);
   for (int c = 0; c < nch; ++c) {
     cyx_ham_s* h = pl->hams[c];
     CYX_CHECK(h->g.nt == nt, "batched channels must share nt (%d vs %d)", h->g.nt, nt);
@@ -1053,6 +1063,8 @@ int cyx_filter_plan_tuning(cyx_filter_plan_t pl, cyx_tuning* t) {
 int cyx_filter_plan_attach_sternheimer(cyx_filter_plan_t pl, cyx_stern_plan_t sp, int iters) {
   CYX_CHECK(pl && (!sp || iters >= 1), "bad argument");
   CYX_CHECK(!sp || stern_plan_context(sp) == pl->ctx, "the Sternheimer plan belongs to another context");
+  CYX_CHECK(!sp || !stern_plan_spectral(sp),
+            "a Sternheimer plan with the spectral preconditioner runs its transforms outside a graph");
   // the next run captures the graph again
   if (pl->exec) {
     CYX_HIP(hipStreamSynchronize(pl->ctx->stream));

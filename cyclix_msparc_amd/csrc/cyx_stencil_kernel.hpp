@@ -57,9 +57,21 @@ template <class T>
 constexpr int stencil_trh(int tr) {
   return tr + (2 * FDN > (int)(256 / sizeof(T)) ? 2 * FDN : (int)(256 / sizeof(T)));
 }
+// Wide theta rings: a workgroup holds tr x nt <= 512 threads, so rings of
+// more than 42 nodes take radial tiles narrower than the 2 FDN halo; a row's
+// thread i then loads positions i + h tr (h = 1..STENCIL_NH_WIDE) of the
+// tile's extended row.  Radial tiles down to STENCIL_TR_MIN (nt <= 128).
+constexpr int STENCIL_TR_MIN = 4;
+static_assert(MAX_NT * STENCIL_TR_MIN <= 512, "theta ring limit and narrowest tile disagree");
+constexpr int STENCIL_NH_WIDE = (2 * FDN + STENCIL_TR_MIN - 1) / STENCIL_TR_MIN;
+constexpr int STENCIL_WIDE = -1;  // the NT value of the narrow-tile variants
 // the compile-time theta count of a launch: 24 or 16 with the tile
-// STENCIL_TR, else 0 (shape read at run time)
-inline int stencil_nt_variant(int nt, int tr) { return (tr == STENCIL_TR && (nt == 24 || nt == 16)) ? nt : 0; }
+// STENCIL_TR, STENCIL_WIDE for radial tiles narrower than the halo, else 0
+// (shape read at run time)
+inline int stencil_nt_variant(int nt, int tr) {
+  if (tr < 2 * FDN) return STENCIL_WIDE;
+  return (tr == STENCIL_TR && (nt == 24 || nt == 16)) ? nt : 0;
+}
 
 // T: arithmetic type of the step; TS: storage type of in/prev/out (T, or
 // the FP32 form of T for an FP64 step over an FP32 block)
@@ -163,11 +175,15 @@ __device__ __forceinline__ void ntstore(cplx<R>* p, cplx<R> v) {
 // channel's launch (the batched kernel runs several channels in one grid).
 // NT > 0: the theta count is NT and the radial tile STENCIL_TR, both known at
 // compile time, so every LDS tap is an immediate offset from one address;
-// NT = 0 reads them from P.
+// NT = 0 reads them from P; NT = STENCIL_WIDE also, with radial tiles below
+// 2 FDN (several right-halo loads per thread).
 template <class T, class TS, int NT, bool HELICAL, bool HAS_PREV, bool HAS_KB>
 __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid0, int nblk, T* lds) {
   using R = typename traits<T>::real;
   constexpr int PP = stencil_pp<T>();
+  // right-halo registers per thread: positions i + h tr_ of the extended row
+  // [r0 - FDN, r0 + tr_ + FDN), h = 1..NH
+  constexpr int NH = NT == STENCIL_WIDE ? STENCIL_NH_WIDE : 1;
   // radial taps from per-thread merged coefficients (FP32 arithmetic: 12
   // registers; FP64 keeps the shared-weight form to save 24 registers)
   constexpr bool MERGED = sizeof(R) == 4;
@@ -241,7 +257,7 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
   const int nplanes = (z1 - z0) + 2 * FDN;
 
   // plane loader: virtual plane kk -> registers (with z Bloch phase)
-  auto load_plane = [&](int kk, T& xa, T& xb, R& ve) {
+  auto load_plane = [&](int kk, T& xa, T (&xb)[NH], R& ve) {
     int kp = kk;
     int wrap = 0;
     if (kp < 0) {
@@ -254,16 +270,21 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
     const TS* row = src + (int64_t)kp * nrt + (int64_t)j * P.nr;
     const int ra = r0 - FDN + i;
     xa = (ra >= 0 && ra < P.nr) ? to_arith<T>(row[ra]) : zero<T>();
-    xb = zero<T>();
-    if (i < 2 * FDN) {
-      const int rb = r0 - FDN + tr_ + i;
-      if (rb < P.nr) xb = to_arith<T>(row[rb]);
+#pragma unroll
+    for (int hh = 0; hh < NH; ++hh) {
+      xb[hh] = zero<T>();
+      const int pos = i + (hh + 1) * tr_;
+      if (pos < tr_ + 2 * FDN) {
+        const int rb = r0 - FDN + pos;
+        if ((NH == 1 || rb >= 0) && rb < P.nr) xb[hh] = to_arith<T>(row[rb]);
+      }
     }
     if constexpr (traits<T>::is_complex) {
       if (wrap != 0) {
         const T ph = wrap < 0 ? P.ph_zl : P.ph_zr;
         xa = cmul(ph, xa);
-        xb = cmul(ph, xb);
+#pragma unroll
+        for (int hh = 0; hh < NH; ++hh) xb[hh] = cmul(ph, xb[hh]);
       }
     }
     ve = (rvalid && kk >= z0 && kk < z1) ? P.veff[(int64_t)kp * nrt + jr] : R(0);
@@ -284,7 +305,7 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
   }
   T pv_cur = zero<T>(), pv_nxt = zero<T>();
 
-  T xa, xb;
+  T xa, xb[NH];
   R ve;
   load_plane(kbeg, xa, xb, ve);
   int buf = 0;
@@ -302,16 +323,22 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
         {
           const int row = (j + FDN) * TRH;
           L[row + i] = xa;
-          if (i < 2 * FDN) L[row + tr_ + i] = xb;
+#pragma unroll
+          for (int hh = 0; hh < NH; ++hh)
+            if (i + hh * tr_ < 2 * FDN) L[row + tr_ + hh * tr_ + i] = xb[hh];
           if (j < FDN) {
             const int rw = (j + FDN + nt_) * TRH;
             L[rw + i] = cmul(P.ph_tr, xa);
-            if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tr, xb);
+#pragma unroll
+            for (int hh = 0; hh < NH; ++hh)
+              if (i + hh * tr_ < 2 * FDN) L[rw + tr_ + hh * tr_ + i] = cmul(P.ph_tr, xb[hh]);
           }
           if (j >= nt_ - FDN) {
             const int rw = (j + FDN - nt_) * TRH;
             L[rw + i] = cmul(P.ph_tl, xa);
-            if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tl, xb);
+#pragma unroll
+            for (int hh = 0; hh < NH; ++hh)
+              if (i + hh * tr_ < 2 * FDN) L[rw + tr_ + hh * tr_ + i] = cmul(P.ph_tl, xb[hh]);
           }
         }
         const R vcur = ve;
@@ -425,19 +452,25 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
   // ---- two planes per barrier (see stencil_pp)
   // stage a plane: interior row, its r halo, and the theta halo copies with
   // the Bloch phase
-  auto stage = [&](T* L, const T& xa, const T& xb) {
+  auto stage = [&](T* L, const T& xa, const T (&xb)[NH]) {
     const int row = (j + FDN) * TRH;
     L[row + i] = xa;
-    if (i < 2 * FDN) L[row + tr_ + i] = xb;
+#pragma unroll
+    for (int hh = 0; hh < NH; ++hh)
+      if (i + hh * tr_ < 2 * FDN) L[row + tr_ + hh * tr_ + i] = xb[hh];
     if (j < FDN) {
       const int rw = (j + FDN + nt_) * TRH;
       L[rw + i] = cmul(P.ph_tr, xa);
-      if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tr, xb);
+#pragma unroll
+      for (int hh = 0; hh < NH; ++hh)
+        if (i + hh * tr_ < 2 * FDN) L[rw + tr_ + hh * tr_ + i] = cmul(P.ph_tr, xb[hh]);
     }
     if (j >= nt_ - FDN) {
       const int rw = (j + FDN - nt_) * TRH;
       L[rw + i] = cmul(P.ph_tl, xa);
-      if (i < 2 * FDN) L[rw + tr_ + i] = cmul(P.ph_tl, xb);
+#pragma unroll
+      for (int hh = 0; hh < NH; ++hh)
+        if (i + hh * tr_ < 2 * FDN) L[rw + tr_ + hh * tr_ + i] = cmul(P.ph_tl, xb[hh]);
     }
   };
   // KB pipeline, per plane slot p of a pair (planes t0+p): hold[p] is the
@@ -486,10 +519,12 @@ __device__ __forceinline__ void stencil_body(const StepParams<T, TS>& P, int bid
       if (z0 + 1 < z1) pv2[1] = to_arith<T>(P.nontemporal ? ntload(pp + nrt) : pp[nrt]);
     }
   }
-  T xa2[2], xb2[2];
+  T xa2[2], xb2[2][NH];
   R ve2[2];
   load_plane(kbeg, xa2[0], xb2[0], ve2[0]);
-  xa2[1] = xb2[1] = zero<T>();
+  xa2[1] = zero<T>();
+#pragma unroll
+  for (int hh = 0; hh < NH; ++hh) xb2[1][hh] = zero<T>();
   ve2[1] = R(0);
   if (nplanes > 1) load_plane(kbeg + 1, xa2[1], xb2[1], ve2[1]);
   int buf = 0;
@@ -655,13 +690,16 @@ void launch_shape(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, bo
 template <class T, class TS>
 void launch_step(const StepParams<T, TS>& P, int nblk, int nthr, size_t shm, bool hel, bool prev, bool kb,
                  hipStream_t st) {
-  // the KB term inside the plane loop exists in the run-time-shaped variant only
+  // the KB term inside the plane loop exists in the run-time-shaped variant
+  // only (the host keeps it out of narrow-tile launches)
   if (kb) return launch_shape<T, TS, 0, true>(P, nblk, nthr, shm, hel, prev, st);
   switch (stencil_nt_variant(P.nt, P.tr)) {
     case 24:
       return launch_shape<T, TS, 24, false>(P, nblk, nthr, shm, hel, prev, st);
     case 16:
       return launch_shape<T, TS, 16, false>(P, nblk, nthr, shm, hel, prev, st);
+    case STENCIL_WIDE:
+      return launch_shape<T, TS, STENCIL_WIDE, false>(P, nblk, nthr, shm, hel, prev, st);
     default:
       return launch_shape<T, TS, 0, false>(P, nblk, nthr, shm, hel, prev, st);
   }

@@ -512,7 +512,8 @@ struct cyx_stern_plan_s {
   const void* PV = nullptr;
   int64_t ldv = 0;
   double tol = 0.0;
-  bool precond = false;
+  bool precond = false;   // Jacobi diagonal of -1/2 Lap (precond = 1)
+  bool spectral = false;  // exact Lap_std^-1 (precond = 2): eager iterations, no graph
   bool owns_ws = false;  // workspace from hipMalloc (public plans) or the handle's buffer (one solve)
   void* ws = nullptr;
   cyx::SternLayout L;
@@ -567,6 +568,15 @@ static int stern_iteration(cyx_stern_plan_s* sp) {
   const int ncol = sp->ncol, nchunk = sp->nchunk, dtype = sp->dtype;
   const double* dinv = sp->precond ? L.dinv : nullptr;
   T* x = (T*)sp->X;
+  // M^-1 z = Lap_std^-1 z (the reference's ILU factors of Lap_std, src/phonon.m:1519)
+  auto spectral = [&](const T* z, T* out) -> int {
+    if constexpr (std::is_same_v<T, double>)
+      return spectral_solve_real(h->spec, z, out, ncol, N, st);
+    else if constexpr (std::is_same_v<T, cplx<double>>)
+      return spectral_solve_complex(h->spec, z, out, ncol, N, st);
+    else
+      return CYX_ERR_ARG;
+  };
   T *r = (T*)L.r, *rt = (T*)L.rt, *p = (T*)L.p, *v = (T*)L.v, *ph = (T*)L.ph, *sh = (T*)L.sh, *t = (T*)L.t;
   D* part = (D*)L.part;
   ColState<D>* S = (ColState<D>*)L.state;
@@ -575,6 +585,7 @@ static int stern_iteration(cyx_stern_plan_s* sp) {
   int rc;
   hipLaunchKernelGGL(bi_p_kernel<T>, gp, dim3(256), 0, st, (const T*)r, p, (const T*)v, ph, dinv, h->g.nr, N,
                      (const ColState<D>*)S);
+  if (sp->spectral && (rc = spectral(p, ph)) != CYX_OK) return rc;
   if ((rc = stern_op<T>(h, dtype, ph, N, v, N, ncol, pv, sp->ldv, sp->nv, L)) != CYX_OK) return rc;
   hipLaunchKernelGGL((coldot_kernel<T, 1>), gc, dim3(AB), 0, st, (const T*)rt, (const T*)v, (const T*)nullptr,
                      (const T*)nullptr, N, N, nchunk, part);
@@ -582,6 +593,7 @@ static int stern_iteration(cyx_stern_plan_s* sp) {
   hipLaunchKernelGGL(bi_s_kernel<T>, gc, dim3(AB), 0, st, r, (const T*)v, sh, dinv, h->g.nr, N, nchunk,
                      (const ColState<D>*)S, part);
   hipLaunchKernelGGL(bi_s_finalize<D>, dim3(ncol), dim3(AB), 0, st, (const D*)part, nchunk, sp->tol, S);
+  if (sp->spectral && (rc = spectral(r, sh)) != CYX_OK) return rc;
   if ((rc = stern_op<T>(h, dtype, sh, N, t, N, ncol, pv, sp->ldv, sp->nv, L)) != CYX_OK) return rc;
   hipLaunchKernelGGL((coldot_kernel<T, 2>), gc, dim3(AB), 0, st, (const T*)t, (const T*)r, (const T*)t,
                      (const T*)t, N, N, nchunk, part);
@@ -619,6 +631,7 @@ int stern_plan_enqueue(cyx_stern_plan_s* sp, hipStream_t st, int iters) {
 }
 
 cyx_ctx_s* stern_plan_context(const cyx_stern_plan_s* sp) { return sp->h->ctx; }
+bool stern_plan_spectral(const cyx_stern_plan_s* sp) { return sp->spectral; }
 
 static void stern_plan_drop_graph(cyx_stern_plan_s* sp) {
   if (sp->exec) (void)hipGraphExecDestroy(sp->exec);
@@ -632,6 +645,10 @@ static void stern_plan_drop_graph(cyx_stern_plan_s* sp) {
 // that count)
 static int stern_plan_run(cyx_stern_plan_s* sp, int iters) {
   cyx_ctx_s* c = sp->h->ctx;
+  if (sp->spectral) {  // rocFFT transforms are launched directly, not captured
+    PhaseTimer tm(c, PH_STERN);
+    return stern_iterations(sp, iters);
+  }
   if (!sp->exec || sp->graph_iters != iters) {
     if (sp->exec) CYX_HIP(hipStreamSynchronize(c->stream));
     stern_plan_drop_graph(sp);
@@ -748,7 +765,16 @@ static int stern_plan_setup(cyx_stern_plan_s* sp, cyx_ham_s* h, int dtype, const
   sp->PV = PV;
   sp->ldv = ldv;
   sp->tol = tol;
-  sp->precond = precond != 0;
+  CYX_CHECK(precond >= 0 && precond <= 2, "precond must be 0 (none), 1 (Jacobi) or 2 (spectral)");
+  sp->precond = precond == 1;
+  sp->spectral = precond == 2;
+  if (sp->spectral) {
+    CYX_CHECK(dtype == CYX_F64 || dtype == CYX_C128, "the spectral preconditioner needs FP64 blocks");
+    if (!h->spec) {
+      const int rc = spectral_create(h->g, &h->spec);
+      if (rc != CYX_OK) return rc;
+    }
+  }
   sp->nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(128, ceil_div(h->N(), 2048)));
   const size_t bytes = stern_plan_layout(sp, nullptr);
   if (own) {

@@ -270,13 +270,32 @@ class Deck:
     meta: dict = field(default_factory=dict)
 
 
-def load_deck(directory, name, cell=None):
+def mesh_to_cheb_degree(h):
+    """Default Chebyshev degree from the effective mesh size
+    (``Mesh2ChebDegree``, ``src/initialization.m:856-869``): a cubic fit in h,
+    14 above h = 0.7, rounded."""
+    if h > 0.7:
+        return 14
+    p3, p2, p1, p0 = -700.0 / 3.0, 1240.0 / 3.0, -773.0 / 3.0, 1078.0 / 15.0
+    return int(_matlab_round(((p3 * h + p2) * h + p1) * h + p0))
+
+
+def scf_tol_for_forces(target_force_accuracy, nelectron, n_atom):
+    """Default SCF tolerance from a force target (``src/initialization.m:515-545``:
+    log10 SCF_tol = (log10 F - b) / a, a = 1.03, b = -0.33145 + 1.1761
+    log10(electrons per atom)); with no accuracy key the target is 1e-3 Ha/Bohr."""
+    b = -0.33145 + 1.1761 * math.log10(nelectron / n_atom)
+    return 10.0 ** ((math.log10(target_force_accuracy) - b) / 1.03)
+
+
+def load_deck(directory, name, cell=None, fd_grid=None):
     """``<directory>/<name>.inpt`` + ``.ion`` + the psp8 it names, for the
     Cyclix decks (one atom type, LDA_PW).  Returns a Deck whose system holds
     the grid, the atoms in (r, theta, z), the psp8 species with its
     pseudocharge radii, NSTATES and CHEB_DEGREE.  ``cell`` = (L1, L2, L3)
     replaces CELL with FD_GRID and the fractional coordinates kept, as a
-    cell relaxation step does (``src/cell_relax.m``)."""
+    cell relaxation step does (``src/cell_relax.m``); ``fd_grid`` = (Nx, Ny,
+    Nz) replaces FD_GRID (intervals, as the key counts them)."""
     d = Path(directory)
     K = read_inpt(d / f"{name}.inpt")
     types = read_ion(d / f"{name}.ion")
@@ -285,8 +304,18 @@ def load_deck(directory, name, cell=None):
     if K.get("EXCHANGE_CORRELATION", ["LDA_PW"])[0] != "LDA_PW":
         raise ValueError("LDA_PW is the supported exchange-correlation functional")
     L1, L2, L3 = (float(v) for v in K["CELL"][:3]) if cell is None else (float(v) for v in cell)
-    nx, ny, nz = (int(v) for v in K["FD_GRID"][:3])
-    bc = [v.lower() for v in K["BC"][:3]]
+    nx, ny, nz = (int(v) for v in (K["FD_GRID"][:3] if fd_grid is None else fd_grid))
+    if "BC" in K:
+        bc = [v.lower() for v in K["BC"][:3]]
+    elif "BOUNDARY_CONDITION" in K:
+        # the obsolete numeric key (src/read_inpt.m:92-96): 5 is the cyclic
+        # sector, 7 the helical one (src/initialization.m:277-298)
+        code = int(K["BOUNDARY_CONDITION"][0])
+        if code not in (5, 7):
+            raise ValueError(f"BOUNDARY_CONDITION {code} is not a cyclic or helical sector")
+        bc = ["d", "c", "p" if code == 5 else "h"]
+    else:
+        raise ValueError("the deck names no boundary condition (BC or BOUNDARY_CONDITION)")
     if bc[0] != "d" or bc[1] != "c" or bc[2] not in ("p", "h"):
         raise ValueError(f"BC {bc} is not a cyclic or helical sector")
     cell_typ = 5 if bc[2] == "h" else 3
@@ -307,18 +336,27 @@ def load_deck(directory, name, cell=None):
     g = GridSpec(nx + 1, ny, nz, xin, dx, dy, dz, twist, fdn, cell_typ)
     psp = Psp8.read(d / t["psp"])
     sp = PspSpecies(psp, t["name"])
-    tol = float(K.get("TOL_PSEUDOCHARGE", ["1e-8"])[0])
+    nelectron = psp.Z * len(atoms)
+    scf_tol = float(K["TOL_SCF"][0]) if "TOL_SCF" in K else scf_tol_for_forces(1e-3, nelectron, len(atoms))
+    # the default pseudocharge tolerance is 0.01 SCF_tol (src/initialization.m:556-560)
+    tol = float(K["TOL_PSEUDOCHARGE"][0]) if "TOL_PSEUDOCHARGE" in K else 0.01 * scf_tol
     sp.rb_xyz = find_rb(sp, g, xmax, xvac, tol)
     nkpt = tuple(int(v) for v in K.get("KPOINT_GRID", ["1", "1", "1"])[:3])
     shift = tuple(float(v) for v in K.get("KPOINT_SHIFT", ["0", "0", "0"])[:3])
     kgrid, w = generate_kpts(nkpt, L1, L2, L3, kptshift=shift, cell_typ=cell_typ)
     channels = [(tuple(float(v) for v in k), float(wk)) for k, wk in zip(kgrid, w)]
-    nstates = int(K["NSTATES"][0])
-    npl = int(K.get("CHEB_DEGREE", ["-1"])[0])
+    if "NSTATES" in K:
+        nstates = int(K["NSTATES"][0])
+    else:   # src/initialization.m:508-513
+        nstates = int(_matlab_round(math.floor(nelectron / 2) * 1.2 + 5))
+    if "CHEB_DEGREE" in K:
+        npl = int(K["CHEB_DEGREE"][0])
+    else:   # src/initialization.m:490-506 on the Cyclix effective mesh size
+        xout = xmax + xvac
+        h_eff = math.sqrt(3.0 / (1 / dx ** 2 + 1 / (((xin + xout) / 2) * dy) ** 2 + 1 / dz ** 2))
+        npl = mesh_to_cheb_degree(h_eff)
     sysm = CyclixSystem(g, atoms, sp, nstates=nstates, npl=npl, kpt=channels[0][0], name=name)
-    opts = {}
-    if "TOL_SCF" in K:
-        opts["SCF_tol"] = float(K["TOL_SCF"][0])
+    opts = {"SCF_tol": scf_tol}
     if "TOL_POISSON" in K:
         opts["poisson_tol"] = float(K["TOL_POISSON"][0])
     ttype = K.get("ELEC_TEMP_TYPE", ["gaussian"])[0].lower()

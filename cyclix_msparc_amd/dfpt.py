@@ -150,7 +150,7 @@ class ResponseResult:
 
 
 def response_scf(gs, u, SCF_tol=1e-5, poisson_tol=None, lin_tol=1e-8, max_scf=100, min_scf=3, lin_maxit=3000,
-                 poisson_maxit=1000, dpsi0=None, drho0=None, proj=None, poll=8, verbose=False):
+                 poisson_maxit=1000, dpsi0=None, drho0=None, proj=None, poll=8, verbose=False, precond="spectral"):
     """The DFPT self-consistency loop of ``src/phonon.m:195-366`` (q = 0,
     potential mixing) for one displacement pattern u (n_atm x 3) on a
     converged FP64 ground state ``gs`` (``scf.GroundState`` after ``run``).
@@ -164,11 +164,15 @@ def response_scf(gs, u, SCF_tol=1e-5, poisson_tol=None, lin_tol=1e-8, max_scf=10
     from the previous solutions (a ``SternheimerPlan``; all ones at the
     start, ``:224``); the first-order occupations (``occupation_pert``,
     host); the density response (``cyx_density_response``); Dphi by
-    Jacobi-AAR on -4 pi (Drho + Db) from the previous Dphi; the output
+    AAR on -4 pi (Drho + Db) from the previous Dphi, with the ground
+    state's Poisson preconditioner; the output
     DVeff = Dphi + fxc Drho (``cyx_lda_pw_fxc``); the error
     ||DVeff_out - DVeff_in|| / ||DVeff_out||; Periodic-Pulay mixing unless
     converged.  SCF_tol 1e-5, Poisson tolerance SCF_tol 1e-3 and the linear
-    solver tolerance 1e-8 are the reference's (``src/phonon.m:11-13``)."""
+    solver tolerance 1e-8 are the reference's (``src/phonon.m:11-13``).
+    ``precond`` is the BiCGSTAB preconditioner where the reference passes the
+    ILU factors of Lap_std (``:1519``): "spectral" (the exact Lap_std^-1),
+    "jacobi" or "none"."""
     import torch
 
     from .api import Hamiltonian, Mixer, SternheimerPlan, lda_pw_fxc
@@ -213,7 +217,8 @@ def response_scf(gs, u, SCF_tol=1e-5, poisson_tol=None, lin_tol=1e-8, max_scf=10
             for k in range(nch):
                 dhams[k].set_veff(dveff_in)
                 _, dlams[k] = dhams[k].sternheimer_rhs(gs.psi[k], nvs[k], coeffs[k], out=B[k])
-                sp = SternheimerPlan(gs.hams[k], B[k], lam[k], gs.psi[k][:nvs[k]], X0=dpsi[k], tol=lin_tol)
+                sp = SternheimerPlan(gs.hams[k], B[k], lam[k], gs.psi[k][:nvs[k]], X0=dpsi[k], tol=lin_tol,
+                                     precond=precond)
                 try:
                     act, info, its, rel = sp.solve(max_iter=lin_maxit, poll=poll)
                     dpsi[k] = sp.X

@@ -94,6 +94,10 @@ class SCFOptions:
     TOL_LANCZOS: float = 1e-2
     lanczos_maxit: int = 300
     MAXIT_POISSON: int = 1000
+    # AAR preconditioner of the Poisson solves: "spectral" (the exact inverse
+    # of Lap_std, in the place of the reference's ILU factors S.LapPreconL/U,
+    # src/poissonSolve.m:36) or "jacobi" (the diagonal of Lap_std)
+    poisson_precond: str = "spectral"
     rhotol: float = 1e-14
     verbose: bool = False
 
@@ -167,7 +171,9 @@ class GroundState:
         td = dev.torch_device
         self.b = torch.from_numpy(np.ascontiguousarray(pc["b"], dtype=float)).to(td)
         self.rho = torch.from_numpy(np.ascontiguousarray(pc["rho_at"], dtype=float)).to(td)
-        self.poisson = Poisson(dev, self.g)
+        if self.opt.poisson_precond not in ("spectral", "jacobi"):
+            raise ValueError(f"poisson_precond {self.opt.poisson_precond!r} is not 'spectral' or 'jacobi'")
+        self.poisson = Poisson(dev, self.g, spectral=self.opt.poisson_precond == "spectral")
         # initial potential (src/scf.m:32-39)
         self.phi, info = self.poisson.solve(self.rho, self.b, None, self.opt.poisson_tol, self.opt.MAXIT_POISSON)
         self.vxc, self.veff = lda_pw(dev, self.rho, self.phi, self.opt.rhotol)

@@ -295,9 +295,11 @@ typedef struct cyx_stern_info {
 
 /* X_j = bicgstab(A_j, B_j, tol, max_iter, M, X_j) for every column at once,
  * A_j the operator above: replaces the per-state bicgstab loop at
- * src/phonon.m:1516-1521.  precond = 1 uses the Jacobi diagonal of
- * -1/2 Lap (per radius) for M (the reference passes the ILU factors of
- * S.Lap_std), 0 uses none.  use_guess = 0 starts from X = 0.
+ * src/phonon.m:1516-1521.  M, where the reference passes the ILU factors
+ * of S.Lap_std (src/initialization.m:752): precond = 2 the exact Lap_std^-1
+ * (theta/z FFT and one band solve per mode, as cyx_poisson_set_spectral;
+ * FP64 blocks; iterations run without graph capture), 1 the Jacobi diagonal
+ * of -1/2 Lap (per radius), 0 none.  use_guess = 0 starts from X = 0.
  * Per-column iteration counts and relative residuals go to col_iters and
  * col_relres (host, may be NULL).  Returns CYX_ERR_NOCONV (with X and info
  * filled) when a column misses tol within max_iter or breaks down. */
@@ -347,7 +349,8 @@ typedef struct cyx_aar_info {
   double relres;      /* ||f - A x|| / ||f|| of the returned x */
   int32_t precond;    /* 0 Jacobi; 1 learned (cyx_poisson_set_preconditioner);
                          2 learned, diverged, solved again with Jacobi (iterations
-                         then count the Jacobi solve) */
+                         then count the Jacobi solve); 3 exact Lap_std^-1
+                         (cyx_poisson_set_spectral) */
   int32_t pinv_eig;   /* Anderson steps whose DF'DF was not certified full rank
                          (lambda_min > 100 m trace eps) and took the
                          eigen-decomposition pseudo-inverse; the others solve
@@ -376,6 +379,21 @@ int cyx_poisson_solve(cyx_poisson_t ps, const double* rho, const double* b, doub
 
 /* y = Lap_std x (device FP64, length N): the operator AAR solves with. */
 int cyx_poisson_apply(cyx_poisson_t ps, const double* x, double* y);
+
+/* on != 0: every later cyx_aar / cyx_poisson_solve on ps preconditions with
+ * the exact inverse of Lap_std in place of the Jacobi diagonal; replaces the
+ * ILU factors S.LapPreconL/U that the reference passes to aar
+ * (src/poissonSolve.m:36, src/initialization.m:752).  Lap_std's theta and z
+ * factors are circulant, so a theta/z Fourier transform splits it into one
+ * band matrix per mode, factored once here (host LU, FP64 device factors of
+ * nr x 13 x nt x nz doubles).  AAR (src/aar.m, the same iteration and
+ * stopping test) then reaches the solution at its first Anderson step:
+ * p + 1 iterations.  info->precond reads 3.  Fails if a band pivot is not
+ * negative (Lap_std not negative definite on the grid). */
+int cyx_poisson_set_spectral(cyx_poisson_t ps, int on);
+/* z = Lap_std^-1 r exactly for ncol real FP64 columns (leading dimension
+ * ld >= N), device memory; builds the factors on first use. */
+int cyx_poisson_spectral_apply(cyx_poisson_t ps, const double* r, double* z, int ncol, int64_t ld);
 
 /* ------------------------------------------------------------ learned Poisson preconditioner */
 /* A linear three-level 3-D convolutional U-Net U on the handle grid (nr, nt,

@@ -576,6 +576,69 @@ def aar(A, b, x_guess, tol, max_iter, omega, beta, m, p, precond):
     return x, relres, count - 1
 
 
+def circulant_symbol(C):
+    """Eigenvalues of a circulant n x n matrix C (C[a, b] = c[(b - a) mod n])
+    on the Fourier vectors numpy's FFT uses: (C x)^(m) = lam[m] x^(m) with
+    lam[m] = sum_d c[d] e^{2 pi i m d / n}."""
+    n = C.shape[0]
+    return np.fft.ifft(C[0, :]) * n
+
+
+def lap_spectral_factors(g: Grid):
+    """S.Lap_std (k = 0) split by the theta/z Fourier transform: its theta and
+    z factors (DL22, DG2, DL33, DG3 of ``src/blochLaplacian_1d.m`` at k = 0)
+    are circulant, so on Fourier mode (m, n) Lap_std is the nr x nr radial
+    matrix A_mn = (DL11 + DG1) + lam22(m) diag(R2inv) + lam33(n) I
+    [+ (lam22(m) + lamG2(m) lamG3(n)) I for helical cells], real for every
+    mode.  Returns the stack A (nt, nz, nr, nr)."""
+    F = bloch_factors(g)
+    l22 = circulant_symbol(F.DL22)
+    l33 = circulant_symbol(F.DL33)
+    Ar = F.DL11 + F.DG1
+    shift = l22[:, None] * 0 + l33[None, :]
+    scale = np.broadcast_to(l22[:, None], (g.nt, g.nz))
+    if F.helical:
+        shift = shift + l22[:, None] + circulant_symbol(F.DG2)[:, None] * circulant_symbol(F.DG3)[None, :]
+    assert np.max(np.abs(np.imag(shift))) <= 1e-9 * np.max(np.abs(shift)) + 1e-300
+    shift, scale = np.real(shift), np.real(scale)
+    A = (Ar[None, None, :, :] + scale[:, :, None, None] * np.diag(F.R2inv)[None, None, :, :]
+         + shift[:, :, None, None] * np.eye(g.nr)[None, None, :, :])
+    return A
+
+
+def lap_spectral_solve(g: Grid, r, A=None, Ainv=None):
+    """x = Lap_std^{-1} r exactly (k = 0): FFT over theta and z, one radial
+    solve per mode (or the product with the per-mode inverses Ainv),
+    inverse FFT.  r: (N,) or (N, ns), real or complex."""
+    r = np.asarray(r)
+    ns = r.shape[1] if r.ndim == 2 else 1
+    R4 = r.reshape(g.nr, g.nt, g.nz, ns, order="F")
+    Rh = np.fft.fft2(R4, axes=(1, 2))                        # (nr, nt, nz, ns)
+    Rh = np.moveaxis(Rh, 0, 2)                                # (nt, nz, nr, ns)
+    if Ainv is not None:
+        Xh = Ainv @ Rh
+    else:
+        Xh = np.linalg.solve(lap_spectral_factors(g) if A is None else A, Rh)
+    X4 = np.fft.ifft2(np.moveaxis(Xh, 2, 0), axes=(1, 2))
+    if not np.iscomplexobj(r):
+        X4 = X4.real
+    out = X4.reshape(g.N, ns, order="F")
+    return out if r.ndim == 2 else out[:, 0]
+
+
+_SPECTRAL_CACHE = {}
+
+
+def spectral_inverses(g: Grid):
+    """The per-mode inverses of lap_spectral_factors(g), kept per grid."""
+    key = (g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
+    if key not in _SPECTRAL_CACHE:
+        if len(_SPECTRAL_CACHE) > 8:
+            _SPECTRAL_CACHE.clear()
+        _SPECTRAL_CACHE[key] = np.linalg.inv(lap_spectral_factors(g))
+    return _SPECTRAL_CACHE[key]
+
+
 def poisson_rhs(rho, b):
     """Cyclix RHS f = -4 pi (rho + b).  ``src/poissonSolve.m:19-23``."""
     return -4 * np.pi * (rho + b)
@@ -588,6 +651,9 @@ def poisson_solve(g: Grid, rho, bcharge, tol, phi_guess=None, maxit=1000, precon
     if precond == "jacobi":
         d = L.diagonal()
         pc = lambda r: r / d
+    elif precond == "spectral":
+        Ainv = spectral_inverses(g)
+        pc = lambda r: lap_spectral_solve(g, r, Ainv=Ainv)
     else:
         ilu = spla.spilu(sp.csc_matrix(L), drop_tol=1e-5)
         pc = ilu.solve
@@ -847,7 +913,7 @@ def electron_density_pert(psis, dpsis, occs, dgs, wkpt):
 def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db, drho0, bet,
              SCF_tol=1e-5, poisson_tol=1e-8, lin_tol=1e-8, max_scf=100, min_scf=3, MINIT_SCF=3,
              hist=7, beta=0.3, omega=None, pfreq=1, lin_maxit=3000, poisson_maxit=1000, rhotol=1e-14,
-             poisson=None, dpsi0=None, linsolve=None):
+             poisson=None, dpsi0=None, linsolve=None, poisson_precond="spectral", stern_precond="spectral"):
     """Self-consistent first-order response to one atomic-displacement
     pattern at q = 0 with potential mixing, ``src/phonon.m:195-366``:
 
@@ -867,7 +933,9 @@ def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db
     atoms, datoms: projector tables and their changes per channel (image
     phases come from each channel's kpt); psis: occupied states per channel
     (N x nv); rho: ground-state density (for fxc).  ``poisson(drho, db, dphi)``
-    overrides the Jacobi-AAR solve and ``linsolve(k, lam, psi, rhs, x0)`` the
+    overrides the AAR solve (preconditioned by ``poisson_precond``: "spectral",
+    the exact Lap_std^-1 that stands in for the reference's ILU factors, or
+    "jacobi") and ``linsolve(k, lam, psi, rhs, x0)`` the
     Sternheimer bicgstab of channel k.  Returns a dict with Drho, Dphi, Dveff
     (the last output potential), Dlambda and Dpsi per channel, DlambdaF,
     count and the error history."""
@@ -876,7 +944,7 @@ def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db
     coeffs = [coeff_pv_psi(ev, oc, bet, psi.shape[1]) for ev, oc, psi in zip(eigvals, occs, psis)]
     if poisson is None:
         def poisson(drho, dbb, guess):
-            x, _, _ = poisson_solve(g, drho, dbb, poisson_tol, guess, poisson_maxit, "jacobi")
+            x, _, _ = poisson_solve(g, drho, dbb, poisson_tol, guess, poisson_maxit, poisson_precond)
             return x
     drho = np.array(drho0, dtype=float, copy=True)
     dphi = np.zeros(g.N)
@@ -895,7 +963,8 @@ def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db
             rhs, dlams[k] = sternheimer_rhs(W, hams[k].kpt, dveff_in, atoms[k], datoms[k], psi, coeffs[k])
             lam = eigvals[k][:psi.shape[1]]
             if linsolve is None:
-                dpsis[k], _, _, _ = sternheimer_solve(hams[k], lam, psi, rhs, dpsis[k], lin_tol, lin_maxit, True)
+                dpsis[k], _, _, _ = sternheimer_solve(hams[k], lam, psi, rhs, dpsis[k], lin_tol, lin_maxit,
+                                                      stern_precond)
             else:
                 dpsis[k] = linsolve(k, lam, psi, rhs, dpsis[k])
         dlf, dgs = occupation_pert([oc[:p.shape[1]] for oc, p in zip(occs, psis)], wkpt, dlams, bet)
@@ -919,12 +988,14 @@ def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db
 def scf(g: Grid, W, atoms, channels, wkpt, bcharge, rho0, Eself, Ecorr, nelec, n_atom, psi0, npl, SCF_tol,
         poisson_tol, maxit_scf=100, minit_scf=3, rho_trigger=4, hist=7, beta=0.3, omega=None, pfreq=1,
         bet=27.211386245988 / 0.2, kind=1, tol_lanczos=1e-2, lanczos_maxit=300, maxit_poisson=1000,
-        rhotol=1e-14):
+        rhotol=1e-14, poisson_precond="spectral"):
     """``src/scf.m:32-306`` for nspin = 1, potential mixing, BC = 3 (no mean
     shift of Veff) and the first relaxation step.  psi0: list of (N, Nev)
-    blocks, one per channel.  Returns a dict with the converged energies,
-    eigenvalues and the per-iteration record."""
-    phi, _, _ = poisson_solve(g, rho0, bcharge, poisson_tol, None, maxit_poisson)
+    blocks, one per channel.  The Poisson solves are AAR preconditioned by
+    ``poisson_precond`` ("spectral": the exact Lap_std^-1, in the place of the
+    reference's ILU factors; or "jacobi").  Returns a dict with the converged
+    energies, eigenvalues and the per-iteration record."""
+    phi, _, _ = poisson_solve(g, rho0, bcharge, poisson_tol, None, maxit_poisson, poisson_precond)
     vxc = lda_pw_vxc(rho0, rhotol)
     veff = phi + vxc                                        # src/scf.m:39
     mix = PeriodicPulay(g.N, hist, beta, omega, pfreq)
@@ -959,7 +1030,7 @@ def scf(g: Grid, W, atoms, channels, wkpt, bcharge, rho0, Eself, Ecorr, nelec, n
         occs = [smearing_occ(e, lam_f, bet, kind) for e in evs]
         if count >= rho_trigger:
             rho = electron_density(psis, occs, wkpt)
-            phi, _, it_p = poisson_solve(g, rho, bcharge, poisson_tol, phi, maxit_poisson)
+            phi, _, it_p = poisson_solve(g, rho, bcharge, poisson_tol, phi, maxit_poisson, poisson_precond)
             vxc = lda_pw_vxc(rho, rhotol)
             veff_out = phi + vxc
             Etot, Eband, Exc, Exc_dc, Eelec_dc, Eent = total_energy(evs, occs, wkpt, lam_f, bet, rho, vxc, phi,
@@ -1045,9 +1116,18 @@ def jacobi_lap_half(g: Grid):
 
 def sternheimer_solve(ham: Hamiltonian, lam, psi_v, B, X0=None, tol=1e-8, maxit=3000, precond=True):
     """Column by column bicgstab on (H - lam_n + psi_v psi_v^H W) x_n = b_n
-    (src/phonon.m:1516-1521).  Returns (X, relres[], iterations[], flags[])."""
-    dinv = jacobi_lap_half(ham.g) if precond else None
-    Minv = (lambda z: dinv * z) if precond else None
+    (src/phonon.m:1516-1521).  precond: "spectral" (the exact Lap_std^-1, in
+    the place of the reference's ILU factors of Lap_std), True / "jacobi"
+    (the diagonal of -1/2 Lap), False / "none".  Returns (X, relres[],
+    iterations[], flags[])."""
+    if precond == "spectral":
+        Ainv = spectral_inverses(ham.g)
+        Minv = lambda z: lap_spectral_solve(ham.g, z, Ainv=Ainv)
+    elif precond is True or precond == "jacobi":
+        dinv = jacobi_lap_half(ham.g)
+        Minv = lambda z: dinv * z
+    else:
+        Minv = None
     ncol = B.shape[1]
     X = np.zeros_like(B)
     rel, its, flags = np.zeros(ncol), np.zeros(ncol, dtype=int), np.zeros(ncol, dtype=int)

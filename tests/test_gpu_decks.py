@@ -1,7 +1,8 @@
 """Whole SCF runs of the reference's own Cyclix decks on the device, against
 the energies those runs printed (``tests/Cyclix/CNT_relax/C.out:86-92``,
-``tests/Cyclix/CNT_ABINITmatch/C.out:82-88``; the decks and numbers are under
-``tests/golden/decks``).
+``tests/Cyclix/CNT_ABINITmatch/C.out:82-88``, ``tests/Cyclix/CNT_chiral/
+C.out_2:82-90`` and ``C.out_1:83-91``, ``LDA_CNT/C.out:84-92``; the decks and
+numbers are under ``tests/golden/decks``).
 
 CNT_relax: (18,0) tube sector, 4 C, ONCV psp8 (s and p projectors), LDA-PW,
 Fermi-Dirac smearing 0.001 Ha, grid 81 x 19 x 32, 12 states, Chebyshev
@@ -10,28 +11,45 @@ relaxation started at 7.98 / 1.2).  CNT_ABINITmatch: (16,0) sector, 4 C,
 grid 101 x 25 x 40, 12 states, degree 100, KPOINT_GRID 1 16 1 (9
 symmetry-adapted Bloch channels with time-reversal weights).
 
-The budget of the energy target is 1e-5 Ha/atom; the runs agree to a few
-1e-8 Ha/atom.  The Fermi level is compared for CNT_ABINITmatch only: CNT_relax
-has a gap, inside which a 0.001 Ha smearing leaves the level undetermined."""
-import json
-from pathlib import Path
+CNT_chiral: a chiral tube as a helical sector (BC D C H, twist 0.163648
+rad/Bohr), 2 C, Fermi-Dirac 0.001 Ha, 6 states, degree 80, at the deck's grid
+141 x 70 x 15 and at 121 x 61 x 8, the grid its C.out_1 run prints (theta
+rings wider than 42 nodes take the narrow-tile stencil; 8 z planes are fewer
+than the z stencil's 13).  LDA_CNT: a (15,0)-like sector, 4 C, 121 x 40 x
+23, 8 symmetry-adapted channels of KPOINT_GRID 1 10 1, Gaussian 0.2 eV, its
+defaults for NSTATES, CHEB_DEGREE and the tolerances (TOL_SCF 1e-4).
 
+The budget of the energy target is 1e-5 Ha/atom; the runs at TOL_SCF 1e-6
+agree to a few 1e-8 Ha/atom.  LDA_CNT's run stopped at TOL_SCF 1e-4 after
+free energies that moved by up to 6e-6 Ha/atom over its last twelve
+iterations (C.out:67-78); its energy is compared at 3e-6 and its band and
+exchange-correlation energies, first order in the potential's error, at
+the size that error allows.  The Fermi level is compared for CNT_ABINITmatch
+only: the other decks have gaps, inside which the smearing leaves the level
+undetermined (CNT_chiral's C.out_1 and C.out_2 runs print -0.064 and -0.065
+Ha, its earlier C.out -0.180)."""
 import pytest
 
 torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
-DECKS = Path(__file__).resolve().parent / "golden" / "decks"
-REF = json.loads((DECKS / "reference_energies.json").read_text())
-RELAX_CELL = (20.0, 0.349065850398866, 7.98 / 1.2)
+from tests.test_oracle_decks import DECKS, REF, RELAX_CELL, load_run  # noqa: E402
+
+# run: (free energy / atom, self + correction, exchange-correlation, band energy) tolerances
+ENERGY_TOL = {
+    "CNT_relax": (1e-6, 1e-6, 1e-5, 1e-4),
+    "CNT_ABINITmatch": (1e-6, 1e-6, 1e-5, 1e-4),
+    "CNT_chiral": (1e-6, 1e-6, 1e-5, 1e-4),
+    "CNT_chiral_coarse": (1e-6, 1e-6, 1e-5, 1e-4),
+    "LDA_CNT": (3e-6, 1e-6, 2e-4, 1e-2),
+}
 
 
-@pytest.mark.parametrize("deck,cell", [("CNT_relax", RELAX_CELL), ("CNT_ABINITmatch", None)])
-def test_reference_deck_energies(device, deck, cell):
-    from cyclix_msparc_amd.deck import load_deck
+@pytest.mark.parametrize("deck", list(ENERGY_TOL))
+def test_reference_deck_energies(device, deck):
     from cyclix_msparc_amd.scf import GroundState, SCFOptions
-    D = load_deck(DECKS / deck, "C", cell=cell)
+    D = load_run(deck)
     s = D.system
     gs = GroundState(device, s, channels=D.channels, options=SCFOptions(**D.options), precision="fp64")
     try:
@@ -46,63 +64,108 @@ def test_reference_deck_energies(device, deck, cell):
         print(f"{deck}: {len(D.channels)} channels, {res.count_SCF} SCF iterations, differences from the "
               f"reference run: " + ", ".join(f"{k} {v:.2e}" for k, v in d.items()), flush=True)
         assert res.converged
-        assert abs(d["Eatom"]) <= 1e-6
-        assert abs(d["Esc"]) <= 1e-6
-        assert abs(d["Exc"]) <= 1e-5
+        te, tsc, txc, tb = ENERGY_TOL[deck]
+        assert abs(d["Eatom"]) <= te
+        assert abs(d["Esc"]) <= tsc
+        assert abs(d["Exc"]) <= txc
         # the band energy moves with the SCF tolerance (1e-6 on the potential)
-        assert abs(d["Eband"]) <= 1e-4
+        assert abs(d["Eband"]) <= tb
         if deck == "CNT_ABINITmatch":
             assert abs(d["Ef"]) <= 1e-5
     finally:
         gs.close()
 
 
-@pytest.mark.parametrize("deck,cell,tol", [("CNT_relax", RELAX_CELL, 1e-6), ("CNT_ABINITmatch", None, 5e-6)])
-def test_reference_deck_forces(device, deck, cell, tol):
+# run: (|F| average / maximum tolerance, per-atom tolerance or None, which printed forces)
+FORCE_TOL = {
+    "CNT_relax": (1e-6, None, None),
+    "CNT_ABINITmatch": (5e-6, 1e-5, None),
+    "LDA_CNT": (1e-4, 2e-4, None),
+    "CNT_chiral_coarse": (1e-5, 1e-5, "earlier_run"),
+}
+
+
+@pytest.mark.parametrize("deck", list(FORCE_TOL))
+def test_reference_deck_forces(device, deck):
     """Drift-free atomic forces (``src/atomicForce.m``,
     ``src/electronicGroundState.m:104-160``) against the reference runs'
-    average and maximum |F| (observed 2.5e-7 Ha/Bohr for CNT_relax; 1.6e-6
-    for CNT_ABINITmatch, whose output comes from an earlier code version),
-    and the per-atom forces CNT_ABINITmatch's C.static holds."""
+    average and maximum |F| and, where a ``.static`` file holds them, the
+    per-atom forces.  Observed: CNT_relax 2.5e-7 Ha/Bohr; CNT_ABINITmatch
+    (an earlier code version) 1.6e-6, per atom 3.5e-6; LDA_CNT 8e-5 per
+    atom, inside the ~2e-4 that its TOL_SCF of 1e-4 allows (the reference's
+    own force-accuracy fit, ``src/initialization.m:534-537``); the helical
+    CNT_chiral at 121 x 61 x 8 matches its earlier run (``C.static``) to
+    1e-6.  Its later run at that grid (``C.static_1``) and the run at the
+    deck's grid (``C.static_2``) agree with these forces in z to 6e-7 but
+    differ in x and y by up to 3.7e-3 and 3.8e-4, with energies that match
+    this method's to 1e-7 and 8e-10 Ha/atom."""
     import numpy as np
 
-    from cyclix_msparc_amd.deck import load_deck
     from cyclix_msparc_amd.forces import atomic_forces, drift_free
     from cyclix_msparc_amd.scf import GroundState, SCFOptions
-    D = load_deck(DECKS / deck, "C", cell=cell)
+    D = load_run(deck)
     s = D.system
     gs = GroundState(device, s, channels=D.channels, options=SCFOptions(**D.options), precision="fp64")
     try:
         assert gs.run().converged
         F = drift_free(atomic_forces(gs), s.grid.cell_typ)
         mag = np.linalg.norm(F, axis=1)
-        ref = REF[deck]
+        tol, tol_atom, which = FORCE_TOL[deck]
+        ref = REF[deck] if which is None else REF[deck][which]
         print(f"{deck}: |F| average {mag.mean():.10f} (reference {ref['average_force']:.10f}), maximum "
               f"{mag.max():.10f} (reference {ref['maximum_force']:.10f})", flush=True)
         assert abs(mag.mean() - ref["average_force"]) <= tol
         assert abs(mag.max() - ref["maximum_force"]) <= tol
-        if "static_forces" in ref:
-            assert np.max(np.abs(F - np.array(ref["static_forces"]))) <= 1e-5
+        if tol_atom is not None:
+            dF = np.max(np.abs(F - np.array(ref["static_forces"])))
+            print(f"{deck}: per-atom forces, max difference {dF:.2e}", flush=True)
+            assert dF <= tol_atom
     finally:
         gs.close()
 
 
-def test_forces_are_the_energy_gradient(device):
-    """Away from the symmetric geometry (atoms moved by a seeded random 0.3 Bohr
-    pattern), F . u equals -dE/du by central differences of the converged
-    free energy along two random directions, to the discretisation gap of the
-    reference's force formula (observed 0.1-0.2 %)."""
+def test_helical_deck_forces_along_the_axis(device):
+    """CNT_chiral at the deck's grid (141 x 70 x 15): the z components of the
+    per-atom forces against its run (``C.static_2``; observed 6e-7)."""
+    import numpy as np
+
+    from cyclix_msparc_amd.forces import atomic_forces, drift_free
+    from cyclix_msparc_amd.scf import GroundState, SCFOptions
+    D = load_run("CNT_chiral")
+    s = D.system
+    gs = GroundState(device, s, channels=D.channels, options=SCFOptions(**D.options), precision="fp64")
+    try:
+        assert gs.run().converged
+        F = drift_free(atomic_forces(gs), s.grid.cell_typ)
+        ref = np.array(REF["CNT_chiral"]["static_forces"])
+        print(f"CNT_chiral forces {F.tolist()}, reference {ref.tolist()}", flush=True)
+        assert np.max(np.abs(F[:, 2] - ref[:, 2])) <= 1e-5
+    finally:
+        gs.close()
+
+
+@pytest.mark.parametrize("deck", ["CNT_relax", "CNT_chiral"])
+def test_forces_are_the_energy_gradient(device, deck):
+    """Away from the symmetric geometry (atoms moved by a seeded random 0.3
+    Bohr pattern), F . u equals -dE/du by central differences of the
+    converged free energy along two random directions, to the discretisation
+    gap of the reference's force formula (observed 0.1-0.2 % on CNT_relax).
+    CNT_chiral is the helical deck (twist 0.1636) at its own grid.  (At the
+    121 x 61 x 8 grid of its C.out_1, 8 z planes under a 13-point stencil,
+    the force formula is far from the energy gradient along z: the z forces
+    of +-0.4368 Ha/Bohr that every run of the reference prints there, and
+    this method reproduces, are not the slope of the energy.)"""
     from dataclasses import replace
 
     import numpy as np
 
     from cyclix_msparc_amd import dfpt
-    from cyclix_msparc_amd.deck import load_deck
     from cyclix_msparc_amd.forces import atomic_forces
     from cyclix_msparc_amd.scf import GroundState, SCFOptions
-    D = load_deck(DECKS / "CNT_relax", "C", cell=RELAX_CELL)
+    D = load_run(deck)
     opts = dict(D.options, SCF_tol=1e-9, poisson_tol=1e-11)
-    s = replace(D.system, atoms=dfpt.displaced_atoms(D.system, dfpt.random_displacements(4, 1, seed=7)[0], 0.3))
+    n = D.system.n_atm
+    s = replace(D.system, atoms=dfpt.displaced_atoms(D.system, dfpt.random_displacements(n, 1, seed=7)[0], 0.3))
 
     def solve(sys_):
         gs = GroundState(device, sys_, channels=D.channels, options=SCFOptions(**opts), precision="fp64")
@@ -114,12 +177,12 @@ def test_forces_are_the_energy_gradient(device):
     F = atomic_forces(gs)
     gs.close()
     for seed in (1, 2):
-        u = dfpt.random_displacements(4, 1, seed=seed)[0]
+        u = dfpt.random_displacements(n, 1, seed=seed)[0]
         E = []
         for eps in (1e-3, -1e-3):
             g2, r2 = solve(replace(s, atoms=dfpt.displaced_atoms(s, u, eps)))
             E.append(r2.Etotal)
             g2.close()
         fu, de = float(np.sum(F * u)), -(E[0] - E[1]) / 2e-3
-        print(f"direction {seed}: F.u {fu:.6f}, -dE/du {de:.6f}", flush=True)
+        print(f"{deck} direction {seed}: F.u {fu:.6f}, -dE/du {de:.6f}", flush=True)
         assert abs(fu - de) <= 4e-3 * abs(de)

@@ -21,6 +21,10 @@ CASES = {
     "cyclic_gamma": (50, 16, 14, 0.0, (0.0, 0.0, 0.0), ("f64", "f32", "c128", "c64")),
     "helical_bloch": (45, 16, 14, 0.05, (0.0, 1.0, 0.6), ("c128", "c64")),
     "cyclic_bloch": (20, 13, 12, 0.0, (0.0, 2.0, 0.0), ("c128",)),
+    # theta rings wider than 42 nodes: radial tiles narrower than the 2 FDN
+    # halo (7 and 5 radii); 8 and 9 z planes, fewer than the 13-point z stencil
+    "helical_wide": (26, 70, 8, 0.05, (0.0, 1.0, 0.6), ("c128", "c64")),
+    "cyclic_wide": (22, 100, 9, 0.0, (0.0, 0.0, 0.0), ("f64", "f32")),
 }
 TOL = {"f64": 1e-11, "c128": 1e-11, "f32": 2e-5, "c64": 2e-5}
 NP = {"f64": np.float64, "c128": np.complex128, "f32": np.float32, "c64": np.complex64}
@@ -102,7 +106,7 @@ def _subspace_projector(psi, Wm, V):
     return psi @ np.linalg.solve(M, psi.conj().T @ (Wm[:, None] * V))
 
 
-@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch"])
+@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch", "helical_wide"])
 def test_eigsolver_pass_parity(device, name):
     from cyclix_msparc_amd.api import Hamiltonian
     sysm, veff, proj, oham, dts = make_case(name)
@@ -134,7 +138,8 @@ def test_eigsolver_pass_parity(device, name):
 
 
 @pytest.mark.parametrize("name,dt", [("cyclic_gamma", "f64"), ("cyclic_gamma", "f32"),
-                                     ("helical_bloch", "c128"), ("helical_bloch", "c64")])
+                                     ("helical_bloch", "c128"), ("helical_bloch", "c64"),
+                                     ("helical_wide", "c64"), ("cyclic_wide", "f32")])
 def test_rayleigh_ritz_many_states(device, name, dt):
     """Rayleigh-Ritz on 70 states: the Gram products cover several 64 x 64
     output regions (Hermitian overlap: upper regions mirrored; H projection:
@@ -252,7 +257,7 @@ def test_chefsi_converges_to_dense_eigenvalues(device, kpt, twist):
     assert np.allclose(ev, lam[:6], atol=1e-8)
 
 
-@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch"])
+@pytest.mark.parametrize("name", ["cyclic_gamma", "helical_bloch", "helical_wide"])
 def test_lanczos_bounds_parity(device, name):
     from cyclix_msparc_amd.api import Hamiltonian
     sysm, veff, proj, oham, dts = make_case(name)
@@ -265,11 +270,11 @@ def test_lanczos_bounds_parity(device, name):
     assert hi == pytest.approx(hi_ref, rel=1e-6)
 
 
-@pytest.mark.parametrize("helical", [False, True])
-def test_aar_poisson_parity(device, helical):
+@pytest.mark.parametrize("helical,nt,nz", [(False, 16, 14), (True, 16, 14), (True, 70, 8)])
+def test_aar_poisson_parity(device, helical, nt, nz):
     from cyclix_msparc_amd.api import Poisson
     from cyclix_msparc_amd.system import GridSpec
-    g = GridSpec(30, 16, 14, 2.0, 0.25, 2 * math.pi / 6 / 16, 0.35, 0.06 if helical else 0.0, 6,
+    g = GridSpec(30, nt, nz, 2.0, 0.25, 2 * math.pi / 6 / nt, 0.35 * 14 / nz, 0.06 if helical else 0.0, 6,
                  5 if helical else 3)
     og = O.Grid(g.nr, g.nt, g.nz, g.xin, g.dr, g.dt, g.dz, g.twist, g.fdn, g.cell_typ)
     rng = np.random.default_rng(8)

@@ -92,7 +92,7 @@ def test_hamiltonian_self_adjoint_in_metric_weights():
 
 # ----------------------------------------------------------------- AAR
 @pytest.mark.parametrize("helical", [False, True])
-@pytest.mark.parametrize("pc", ["jacobi", "ilu"])
+@pytest.mark.parametrize("pc", ["jacobi", "ilu", "spectral"])
 def test_aar_solves_lap_std(helical, pc):
     g = O.Grid(16, 14, 13, 2.0, 0.3, 2 * math.pi / 6 / 14, 0.35, 0.07 if helical else 0.0, 6, 5 if helical else 3)
     L = O.lap_matrix(g)
@@ -104,6 +104,52 @@ def test_aar_solves_lap_std(helical, pc):
     xd = spla.spsolve(L.tocsc(), f)
     assert relres <= 1e-10
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
+
+
+# grids: cyclic; helical; a helical ring wider than the stencil's 42-node
+# tiles with 8 z planes (the 13-point z stencil folds onto itself)
+SPECTRAL_GRIDS = [(16, 14, 13, 0.0, 3), (16, 14, 13, 0.07, 5), (12, 70, 8, 0.16, 5)]
+
+
+@pytest.mark.parametrize("nr,nt,nz,tw,ct", SPECTRAL_GRIDS)
+def test_spectral_inverse_of_lap_std(nr, nt, nz, tw, ct):
+    """The theta/z Fourier split of Lap_std (its theta and z factors are
+    circulant) inverts the assembled sparse operator exactly: Lap_std z = r
+    to rounding, and z equals a sparse direct solve; real and complex
+    right-hand sides.  The per-mode matrices are real and their no-pivot
+    LU pivots all negative (Lap_std negative definite)."""
+    g = O.Grid(nr, nt, nz, 2.0, 0.3, 2 * math.pi / 6 / nt, 0.35, tw, 6, ct)
+    L = O.lap_matrix(g).tocsc()
+    rng = np.random.default_rng(11)
+    R = rng.standard_normal((g.N, 2)) + 1j * rng.standard_normal((g.N, 2))
+    Z = O.lap_spectral_solve(g, R)
+    assert np.linalg.norm(L @ Z - R) / np.linalg.norm(R) < 1e-12
+    Zd = spla.spsolve(L, R)
+    assert np.linalg.norm(Z - Zd) / np.linalg.norm(Zd) < 1e-10
+    zr = O.lap_spectral_solve(g, R[:, 0].real)
+    assert not np.iscomplexobj(zr) and np.linalg.norm(L @ zr - R[:, 0].real) < 1e-12 * np.linalg.norm(R[:, 0].real)
+    A = O.lap_spectral_factors(g)
+    for a in A.reshape(-1, nr, nr)[:: max(1, nt * nz // 17)]:
+        U = sla.lu(a, permute_l=False)[2]
+        assert np.all(np.linalg.eigvals(a).real < 0)
+        assert np.all(np.diag(sla.lu_factor(a)[0]) != 0) and U.shape == a.shape
+
+
+def test_spectral_aar_reaches_solution_at_first_anderson_step():
+    """With M = Lap_std^-1 the preconditioned residual is the error, the
+    Richardson steps shrink it by 1 - omega, and the first Anderson step
+    (count = p) lands on the solution: relres at count p + 1 is rounding."""
+    g = O.Grid(16, 14, 13, 2.0, 0.3, 2 * math.pi / 6 / 14, 0.35, 0.07, 6, 5)
+    rng = np.random.default_rng(3)
+    rho = np.abs(rng.standard_normal(g.N))
+    b = -rho.mean() * np.ones(g.N)
+    for p in (2, 6):
+        L = O.lap_matrix(g)
+        A = O.lap_spectral_factors(g)
+        f = O.poisson_rhs(rho, b)
+        x, relres, it = O.aar(lambda v: L @ v, f, None, 1e-13, 100, 0.6, 0.6, 7, p,
+                              lambda r: O.lap_spectral_solve(g, r, A))
+        assert it == p + 1 and relres < 1e-13
 
 
 def test_aar_manufactured_potential():
