@@ -913,7 +913,7 @@ def electron_density_pert(psis, dpsis, occs, dgs, wkpt):
 def dfpt_scf(g: Grid, W, hams, atoms, datoms, psis, eigvals, occs, wkpt, rho, db, drho0, bet,
              SCF_tol=1e-5, poisson_tol=1e-8, lin_tol=1e-8, max_scf=100, min_scf=3, MINIT_SCF=3,
              hist=7, beta=0.3, omega=None, pfreq=1, lin_maxit=3000, poisson_maxit=1000, rhotol=1e-14,
-             poisson=None, dpsi0=None, linsolve=None, poisson_precond="spectral", stern_precond="spectral"):
+             poisson=None, dpsi0=None, linsolve=None, poisson_precond="spectral", stern_precond="jacobi"):
     """Self-consistent first-order response to one atomic-displacement
     pattern at q = 0 with potential mixing, ``src/phonon.m:195-366``:
 
