@@ -217,52 +217,78 @@ void spectral_destroy(Spectral* S) {
 size_t spectral_factor_bytes(const Spectral* S) { return (size_t)S->nr * NBD * S->nmode * sizeof(double); }
 
 // forward and back substitution of every mode, in place on the mode-major
-// spectrum Y ([i][mode], ncol columns of nr x nmode), scaled by `scale`
-__global__ void __launch_bounds__(256) band_solve_kernel(const double* __restrict__ lu, cplx<double>* __restrict__ Y,
+// spectrum Y ([i][mode], ncol columns of nr x nmode), scaled by `scale`.
+// One thread per mode; the rows go in chunks of BSU whose factor entries and
+// right-hand sides are all loaded before the chunk's recurrence, so a thread
+// waits on memory once per chunk instead of once per row.
+constexpr int BSU = 8;
+constexpr int BSB = 64;  // threads per block: more blocks for few modes
+__global__ void __launch_bounds__(BSB) band_solve_kernel(const double* __restrict__ lu, cplx<double>* __restrict__ Y,
                                                          int nr, int nmode, double scale) {
-  const int mode = blockIdx.x * 256 + threadIdx.x;
+  const int mode = blockIdx.x * BSB + threadIdx.x;
   if (mode >= nmode) return;
   cplx<double>* y = Y + (int64_t)blockIdx.y * nr * nmode + mode;
   const int64_t rs = nmode;                  // stride between radii
   const int64_t ds = nmode;                  // stride between stored diagonals
   const int64_t ls = (int64_t)NBD * nmode;   // stride between rows of the factors
-  // forward: y_i -= sum_{d=1..BW} l_{i,i-d} y_{i-d}; the last BW values in registers
+  const double* lm = lu + mode;
+  // forward: y_i -= sum_{d=1..BW} l_{i,i-d} y_{i-d}; w[BW - d] holds y_{i-d}
   cplx<double> w[BW];
 #pragma unroll
   for (int d = 0; d < BW; ++d) w[d] = cplx<double>{0.0, 0.0};
-  for (int i = 0; i < nr; ++i) {
-    const double* l = lu + i * ls + mode;
-    cplx<double> v = y[i * rs];
+  for (int i0 = 0; i0 < nr; i0 += BSU) {
+    double c[BSU][BW];
+    cplx<double> b[BSU];
 #pragma unroll
-    for (int d = 1; d <= BW; ++d) {  // w[BW - d] holds y_{i-d}
-      const double c = l[(BW - d) * ds];
-      v.x -= c * w[BW - d].x;
-      v.y -= c * w[BW - d].y;
+    for (int u = 0; u < BSU; ++u) {
+      const int i = i0 + u;
+      b[u] = i < nr ? y[i * rs] : cplx<double>{0.0, 0.0};
+#pragma unroll
+      for (int d = 1; d <= BW; ++d) c[u][d - 1] = i < nr ? lm[i * ls + (BW - d) * ds] : 0.0;
     }
 #pragma unroll
-    for (int d = 0; d < BW - 1; ++d) w[d] = w[d + 1];
-    w[BW - 1] = v;
-    y[i * rs] = v;
+    for (int u = 0; u < BSU; ++u) {
+      cplx<double> v = b[u];
+#pragma unroll
+      for (int d = 1; d <= BW; ++d) {
+        v.x -= c[u][d - 1] * w[BW - d].x;
+        v.y -= c[u][d - 1] * w[BW - d].y;
+      }
+#pragma unroll
+      for (int d = 0; d < BW - 1; ++d) w[d] = w[d + 1];
+      w[BW - 1] = v;
+      if (i0 + u < nr) y[(i0 + u) * rs] = v;
+    }
   }
-  // back: x_i = (y_i - sum_{d=1..BW} u_{i,i+d} x_{i+d}) / u_ii
+  // back: x_i = (y_i - sum_{d=1..BW} u_{i,i+d} x_{i+d}) / u_ii; w[d - 1] holds x_{i+d}
 #pragma unroll
   for (int d = 0; d < BW; ++d) w[d] = cplx<double>{0.0, 0.0};
-  for (int i = nr - 1; i >= 0; --i) {
-    const double* u = lu + i * ls + mode;
-    cplx<double> v = y[i * rs];
+  for (int i0 = nr - 1; i0 >= 0; i0 -= BSU) {
+    double c[BSU][BW], inv[BSU];
+    cplx<double> b[BSU];
 #pragma unroll
-    for (int d = 1; d <= BW; ++d) {  // w[d - 1] holds x_{i+d}
-      const double c = u[(BW + d) * ds];
-      v.x -= c * w[d - 1].x;
-      v.y -= c * w[d - 1].y;
+    for (int u = 0; u < BSU; ++u) {
+      const int i = i0 - u;
+      b[u] = i >= 0 ? y[i * rs] : cplx<double>{0.0, 0.0};
+      inv[u] = i >= 0 ? lm[i * ls + BW * ds] : 0.0;
+#pragma unroll
+      for (int d = 1; d <= BW; ++d) c[u][d - 1] = i >= 0 ? lm[i * ls + (BW + d) * ds] : 0.0;
     }
-    const double inv = u[BW * ds];
-    v.x *= inv;
-    v.y *= inv;
 #pragma unroll
-    for (int d = BW - 1; d > 0; --d) w[d] = w[d - 1];
-    w[0] = v;
-    y[i * rs] = cplx<double>{v.x * scale, v.y * scale};
+    for (int u = 0; u < BSU; ++u) {
+      cplx<double> v = b[u];
+#pragma unroll
+      for (int d = 1; d <= BW; ++d) {
+        v.x -= c[u][d - 1] * w[d - 1].x;
+        v.y -= c[u][d - 1] * w[d - 1].y;
+      }
+      v.x *= inv[u];
+      v.y *= inv[u];
+#pragma unroll
+      for (int d = BW - 1; d > 0; --d) w[d] = w[d - 1];
+      w[0] = v;
+      if (i0 - u >= 0) y[(i0 - u) * rs] = cplx<double>{v.x * scale, v.y * scale};
+    }
   }
 }
 
@@ -300,7 +326,7 @@ static int solve_inplace_complex(Spectral* S, cplx<double>* R, cplx<double>* mid
     void* out[1] = {mid + (int64_t)c * S->N};
     CYX_FFT(rocfft_execute(S->fwd, in, out, S->info));
   }
-  hipLaunchKernelGGL(band_solve_kernel, dim3(ceil_div(S->nmode, 256), ncol), dim3(256), 0, st, (const double*)S->lu,
+  hipLaunchKernelGGL(band_solve_kernel, dim3(ceil_div(S->nmode, BSB), ncol), dim3(BSB), 0, st, (const double*)S->lu,
                      mid, S->nr, S->nmode, scale);
   CYX_HIP(hipGetLastError());
   for (int c = 0; c < ncol; ++c) {
