@@ -388,8 +388,9 @@ int cyx_poisson_apply(cyx_poisson_t ps, const double* x, double* y);
  * band matrix per mode, factored once here (host LU, FP64 device factors of
  * nr x 13 x nt x nz doubles).  AAR (src/aar.m, the same iteration and
  * stopping test) then reaches the solution at its first Anderson step:
- * p + 1 iterations.  info->precond reads 3.  Fails if a band pivot is not
- * negative (Lap_std not negative definite on the grid). */
+ * p + 1 iterations.  info->precond reads 3.  It takes precedence over a
+ * learned preconditioner set with cyx_poisson_set_preconditioner.  Fails if
+ * a band pivot is not negative (Lap_std not negative definite on the grid). */
 int cyx_poisson_set_spectral(cyx_poisson_t ps, int on);
 /* z = Lap_std^-1 r exactly for ncol real FP64 columns (leading dimension
  * ld >= N), device memory; builds the factors on first use. */
