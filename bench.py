@@ -6,16 +6,18 @@ Workload: (10,0) carbon nanotube, 1/5 sector with helical twist 0.05 rad/Bohr,
 120 x 24 x 96 (N = 276,480), one complex Bloch channel (nu = 1,
 eta = 0.25 * 2pi/L3), 256 states, Chebyshev degree 24.
 
-Two paths are timed: the FP64 path (complex128 throughout, Jacobi-AAR) and
+Two paths are timed: the FP64 path (complex128 throughout) and
 the mixed path (complex64 psi and Chebyshev filter; the Rayleigh-Ritz H psi
 in FP64 arithmetic, FP64 accumulation of the Gram products, FP64 dense
 eigensolve, density and energies; with --learned also the learned Poisson
-correction).  `value` is the mixed path's time per step; `speedup_vs_fp64`
+correction).  Both paths' Poisson solves are AAR with the exact inverse of
+Lap_std as the preconditioner (--poisson-precond jacobi: its diagonal).
+`value` is the mixed path's time per step; `speedup_vs_fp64`
 compares the two.  `error_vs_fp64` compares the two paths' converged
 energies, and their occupied eigenvalues after both final blocks take more
 CheFSI passes on the FP64 path's converged potential.  The standalone Poisson
-solve is timed with Jacobi-AAR and, when trained weights for the grid exist,
-with the learned correction.
+solve is timed with both preconditioners and, when trained weights for the
+grid exist, with the learned correction on Jacobi-AAR.
 
 One step = one SCF iteration of src/scf.m on the device, at the SCF_tol of
 a 1e-5 Ha/atom energy target (src/initialization.m:516-552): CheFSI pass
@@ -70,7 +72,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="cnt10_helical")
     p.add_argument("--paths", default="fp64,mixed",
-                   help="comma list of fp64 (complex128, Jacobi-AAR) and mixed (complex64 filter and subspace "
+                   help="comma list of fp64 (complex128) and mixed (complex64 filter and subspace "
                         "inputs with FP64 accumulation, learned Poisson correction when weights exist)")
     p.add_argument("--learned", action="store_true",
                    help="use the learned Poisson correction inside the mixed path's SCF steps (it is measured "

@@ -17,7 +17,7 @@ tables, ``CyclixSystem.pseudocharge_derivative``,
 ``atomic_density_derivative`` and ``projector_derivatives``), the
 coefficient matrix coeff (``:112-158``).  On the device: the right-hand
 sides (``cyx_sternheimer_rhs``), the batched solves (``SternheimerPlan``),
-the density response (``cyx_density_response``), Dphi (Jacobi-AAR), the
+the density response (``cyx_density_response``), Dphi (AAR), the
 exchange-correlation kernel (``cyx_lda_pw_fxc``) and the mixing.
 """
 from __future__ import annotations
