@@ -142,7 +142,6 @@ namespace cyx {
 struct Spectral;
 int spectral_create(const cyx_grid_desc& g, Spectral** out);
 void spectral_destroy(Spectral* S);
-size_t spectral_factor_bytes(const Spectral* S);
 // z = Lap_std^-1 r for ncol columns (leading dimension ld) on stream st
 int spectral_solve_real(Spectral* S, const double* r, double* z, int ncol, int64_t ld, hipStream_t st);
 int spectral_solve_complex(Spectral* S, const cplx<double>* r, cplx<double>* z, int ncol, int64_t ld, hipStream_t st);

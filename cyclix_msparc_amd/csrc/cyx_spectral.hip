@@ -214,7 +214,6 @@ void spectral_destroy(Spectral* S) {
   delete S;
 }
 
-size_t spectral_factor_bytes(const Spectral* S) { return (size_t)S->nr * NBD * S->nmode * sizeof(double); }
 
 // forward and back substitution of every mode, in place on the mode-major
 // spectrum Y ([i][mode], ncol columns of nr x nmode), scaled by `scale`.
