@@ -194,7 +194,7 @@ def test_oracle_scf_matches_the_reference_run():
 
 
 @pytest.mark.skipif(not __import__("os").environ.get("CYX_SLOW_ORACLE"),
-                    reason="about 30 minutes on 8 cores (set CYX_SLOW_ORACLE=1); measured 2.8e-8 Ha/atom")
+                    reason="about 11 minutes on 8 cores (set CYX_SLOW_ORACLE=1); measured 2.8e-8 Ha/atom")
 def test_oracle_scf_matches_the_helical_reference_run():
     """The oracle's whole SCF on the helical CNT_chiral deck (141 x 70 x 15,
     twist 0.1636, Fermi-Dirac 0.001 Ha) against the free energy per atom of
