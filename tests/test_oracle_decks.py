@@ -210,3 +210,25 @@ def test_oracle_scf_matches_the_helical_reference_run():
                 s.nelectron, s.n_atm, [psi0_for(s)], s.npl, o.SCF_tol, o.poisson_tol, bet=o.bet, kind=o.elec_T_type)
     assert res["converged"]
     assert abs(res["Eatom"] - REF["CNT_chiral"]["free_energy_per_atom"]) <= 1e-6
+
+
+def test_numeric_boundary_codes(tmp_path):
+    """BOUNDARY_CONDITION 5 is the cyclic sector and 7 the helical one
+    (``src/initialization.m:287-295``); the other codes are not Cyclix
+    sectors and are refused, as is a deck with no boundary key."""
+    import shutil
+    src = DECKS / "LDA_CNT"
+    for f in ("C.ion", "C.psp8"):
+        shutil.copy(src / f, tmp_path / f)
+    text = (src / "C.inpt").read_text()
+    for code, ok in ((5, True), (2, False), (4, False), (None, False)):
+        body = text.replace("BOUNDARY_CONDITION: 5", "" if code is None else f"BOUNDARY_CONDITION: {code}")
+        (tmp_path / "C.inpt").write_text(body)
+        if ok:
+            assert load_deck(tmp_path, "C").system.grid.cell_typ == 3
+        else:
+            with pytest.raises(ValueError):
+                load_deck(tmp_path, "C")
+    (tmp_path / "C.inpt").write_text(text.replace("BOUNDARY_CONDITION: 5", "BOUNDARY_CONDITION: 7\nTWIST_ANGLE: 0.01"))
+    g = load_deck(tmp_path, "C").system.grid
+    assert g.cell_typ == 5 and g.twist == pytest.approx(0.01)
